@@ -1,0 +1,282 @@
+// sht_capi.hip -- the extern "C" entry points of libsht_hip.so (include/sht.h).
+#include <cstring>
+#include <new>
+
+#include "sht_internal.h"
+
+namespace sht {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+static int validate_desc(const sht_desc *d) {
+  if (!d) return fail(SHT_ERR_INVALID, "sht_plan_create: desc is NULL");
+  if (d->nmax < 0 || d->nmax > 255)
+    return fail(SHT_ERR_INVALID, "sht_plan_create: nmax must be in [0, 255]");
+  if (d->jm < 2 || (d->jm & 1))
+    return fail(SHT_ERR_INVALID, "sht_plan_create: jm must be even and >= 2");
+  if (d->jm <= d->nmax)
+    return fail(SHT_ERR_INVALID, "sht_plan_create: jm must exceed nmax (exact quadrature)");
+  if (d->im < 2 * d->nmax + 2)
+    return fail(SHT_ERR_INVALID, "sht_plan_create: im must be >= 2*nmax + 2");
+  if (!is_pow2(d->im) || d->im < 16 || d->im > 2048)
+    return fail(SHT_ERR_UNSUPPORTED, "sht_plan_create: im must be a power of two in [16, 2048]");
+  if (d->kmax < 1) return fail(SHT_ERR_INVALID, "sht_plan_create: kmax must be >= 1");
+  if (d->precision < SHT_PREC_FP64 || d->precision > SHT_PREC_SCALED_FP16)
+    return fail(SHT_ERR_INVALID, "sht_plan_create: unknown precision");
+  if (d->flags & ~SHT_FLAG_GRAD) return fail(SHT_ERR_INVALID, "sht_plan_create: unknown flags");
+  const size_t four = (size_t)(d->jm / 2) * (d->nmax + 1) * 4 * (size_t)d->kmax;
+  if (four > ((size_t)1 << 34))
+    return fail(SHT_ERR_INVALID, "sht_plan_create: kmax too large for the Fourier scratch");
+  return SHT_OK;
+}
+
+static void release(Plan &p) {
+  void *ptrs[] = {p.mu, p.w, p.coslat, p.twid, p.Pinv, p.Hinv, p.Pfwd, p.poff_inv, p.poff_fwd,
+                  p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.four, p.four2, p.stage_a, p.stage_b};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  if (p.pinned) (void)hipHostFree(p.pinned);
+  if (p.host_stream) (void)hipStreamDestroy(p.host_stream);
+  p = Plan{};
+}
+
+static int check_call(const sht_plan *h, const void *a, const void *b, int nf, const char *who) {
+  if (!h) return fail(SHT_ERR_INVALID, std::string(who) + ": plan is NULL");
+  if (!a || !b) return fail(SHT_ERR_INVALID, std::string(who) + ": NULL array");
+  if (nf < 1 || nf > h->p.kmax)
+    return fail(SHT_ERR_INVALID, std::string(who) + ": nfield must be in [1, kmax]");
+  return SHT_OK;
+}
+
+static int ensure_host_staging(Plan &p) {
+  if (p.pinned) return SHT_OK;
+  const size_t grid = (size_t)p.kmax * p.jm * p.im;
+  const size_t wa = (size_t)p.kmax * (p.N + 1) * (p.N + 1);
+  const size_t spec = (size_t)p.kmax * p.K * 2;
+  const size_t big = grid > wa ? grid : wa;
+  SHT_HIP(hipStreamCreateWithFlags(&p.host_stream, hipStreamNonBlocking));
+  SHT_HIP(hipMalloc(&p.stage_a, sizeof(double) * big));
+  SHT_HIP(hipMalloc(&p.stage_b, sizeof(double) * spec));
+  SHT_HIP(hipHostMalloc(&p.pinned, sizeof(double) * big, hipHostMallocDefault));
+  p.pinned_bytes = sizeof(double) * big;
+  return SHT_OK;
+}
+
+static int spec_to_grid(Plan &p, const double *spec, double *grid, int nf, hipStream_t s) {
+  int rc;
+  if (p.prec == SHT_PREC_FP64)
+    rc = launch_leg_inv_f64(p, spec, p.four, p.four2, true, false, nf, s);
+  else
+    rc = launch_leg_inv_mixed(p, spec, p.four, nf, s);
+  if (rc) return rc;
+  return launch_fft_inv(p, p.four, nullptr, grid, nullptr, nullptr, nf, s);
+}
+
+static int grid_to_spec(Plan &p, const double *grid, double *spec, int nf, hipStream_t s) {
+  int rc = launch_fft_fwd(p, grid, p.four, nf, s);
+  if (rc) return rc;
+  if (p.prec == SHT_PREC_FP64) return launch_leg_fwd_f64(p, p.four, spec, nf, s);
+  return launch_leg_fwd_mixed(p, p.four, spec, nf, s);
+}
+
+// Gradients always run the FP64 Legendre step (the derivative table is FP64 only).
+static int spec_to_grad(Plan &p, const double *spec, double *grid, double *glon, double *glat,
+                        int nf, hipStream_t s) {
+  const bool value_tab = grid || glon;
+  const bool deriv_tab = glat != nullptr;
+  int rc = launch_leg_inv_f64(p, spec, p.four, p.four2, value_tab, deriv_tab, nf, s);
+  if (rc) return rc;
+  return launch_fft_inv(p, p.four, p.four2, grid, glon, glat, nf, s);
+}
+
+}  // namespace sht
+
+using namespace sht;
+
+extern "C" {
+
+const char *sht_last_error(void) { return g_err.c_str(); }
+int sht_abi_version(void) { return SHT_ABI_VERSION; }
+
+int sht_plan_create(const sht_desc *desc, sht_plan **out) {
+  if (!out) return fail(SHT_ERR_INVALID, "sht_plan_create: out is NULL");
+  *out = nullptr;
+  int rc = validate_desc(desc);
+  if (rc) return rc;
+  sht_plan *h = new (std::nothrow) sht_plan;
+  if (!h) return fail(SHT_ERR_NOMEM, "sht_plan_create: host allocation failed");
+  Plan &p = h->p;
+  p.N = desc->nmax;
+  p.im = desc->im;
+  p.jm = desc->jm;
+  p.P2 = desc->jm / 2;
+  p.P2p = roundup(p.P2, 16);
+  p.K = (p.N + 1) * (p.N + 2) / 2;
+  p.kmax = desc->kmax;
+  p.prec = desc->precision;
+  p.flags = desc->flags;
+  p.maxke = ceil_half(p.N + 1);
+  hipError_t e = hipGetDevice(&p.device);
+  if (e != hipSuccess) {
+    delete h;
+    return fail(SHT_ERR_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  }
+  rc = build_tables(p);
+  if (!rc && p.prec != SHT_PREC_FP64) rc = build_mixed_tables(p);
+  if (!rc) {
+    const size_t four = (size_t)p.P2 * (p.N + 1) * 4 * (size_t)p.kmax;
+    e = hipMalloc(&p.four, sizeof(double) * four);
+    if (e == hipSuccess && (p.flags & SHT_FLAG_GRAD)) e = hipMalloc(&p.four2, sizeof(double) * four);
+    if (e != hipSuccess) rc = fail(SHT_ERR_NOMEM, "sht_plan_create: Fourier scratch allocation failed");
+  }
+  if (rc) {
+    std::string keep = g_err;
+    release(p);
+    delete h;
+    g_err = keep;
+    return rc;
+  }
+  *out = h;
+  return SHT_OK;
+}
+
+void sht_plan_destroy(sht_plan *h) {
+  if (!h) return;
+  release(h->p);
+  delete h;
+}
+
+int sht_plan_grid(const sht_plan *h, double *lon, double *lat, double *lon_weight,
+                  double *lat_weight) {
+  if (!h) return fail(SHT_ERR_INVALID, "sht_plan_grid: plan is NULL");
+  const Plan &p = h->p;
+  for (int i = 0; i < p.im; ++i) {
+    if (lon) lon[i] = 2.0 * M_PI * (double)i / (double)p.im;
+    if (lon_weight) lon_weight[i] = 2.0 * M_PI / (double)p.im;
+  }
+  for (int j = 0; j < p.jm; ++j) {
+    if (lat) lat[j] = (p.h_mu[j] < 0 ? -1.0 : 1.0) * acos(p.h_coslat[j] > 1.0 ? 1.0 : p.h_coslat[j]);
+    if (lat_weight) lat_weight[j] = p.h_w[j];
+  }
+  return SHT_OK;
+}
+
+int sht_l_nm(int n, int m) {
+  if (n < 0 || m < -n || m > n) return fail(SHT_ERR_INVALID, "sht_l_nm: need 0 <= |m| <= n");
+  return n * (n + 1) + m + 1;
+}
+
+int sht_nm_l(int l, int *n, int *m) {
+  if (l < 1 || !n || !m) return fail(SHT_ERR_INVALID, "sht_nm_l: need l >= 1");
+  int nn = 0;
+  while ((nn + 1) * (nn + 1) < l) ++nn;
+  *n = nn;
+  *m = l - 1 - nn * (nn + 1);
+  return SHT_OK;
+}
+
+int sht_spec_to_grid_dev(sht_plan *h, const double *spec, double *grid, int nf, void *stream) {
+  int rc = check_call(h, spec, grid, nf, "sht_spec_to_grid_dev");
+  if (rc) return rc;
+  return spec_to_grid(h->p, spec, grid, nf, (hipStream_t)stream);
+}
+
+int sht_grid_to_spec_dev(sht_plan *h, const double *grid, double *spec, int nf, void *stream) {
+  int rc = check_call(h, grid, spec, nf, "sht_grid_to_spec_dev");
+  if (rc) return rc;
+  return grid_to_spec(h->p, grid, spec, nf, (hipStream_t)stream);
+}
+
+int sht_spec_to_grad_dev(sht_plan *h, const double *spec, double *grid, double *gradlon,
+                         double *gradlat, int nf, void *stream) {
+  double *any = grid ? grid : (gradlon ? gradlon : gradlat);
+  int rc = check_call(h, spec, any, nf, "sht_spec_to_grad_dev");
+  if (rc) return rc;
+  if (gradlat && !(h->p.flags & SHT_FLAG_GRAD))
+    return fail(SHT_ERR_INVALID, "sht_spec_to_grad_dev: GradLat needs a plan built with SHT_FLAG_GRAD");
+  return spec_to_grad(h->p, spec, grid, gradlon, gradlat, nf, (hipStream_t)stream);
+}
+
+int sht_spml_to_canon_dev(const sht_plan *h, const double *wa, double *spec, int nf, void *stream) {
+  int rc = check_call(h, wa, spec, nf, "sht_spml_to_canon_dev");
+  if (rc) return rc;
+  return launch_spml_to_canon(h->p, wa, spec, nf, (hipStream_t)stream);
+}
+
+int sht_canon_to_spml_dev(const sht_plan *h, const double *spec, double *wa, int nf, void *stream) {
+  int rc = check_call(h, spec, wa, nf, "sht_canon_to_spml_dev");
+  if (rc) return rc;
+  return launch_canon_to_spml(h->p, spec, wa, nf, (hipStream_t)stream);
+}
+
+int sht_avr_lonlat_dev(const sht_plan *h, const double *grid, double *avg, int nf, void *stream) {
+  if (!h) return fail(SHT_ERR_INVALID, "sht_avr_lonlat_dev: plan is NULL");
+  if (!grid || !avg || nf < 1) return fail(SHT_ERR_INVALID, "sht_avr_lonlat_dev: bad arguments");
+  return launch_avr_lonlat(h->p, grid, avg, nf, (hipStream_t)stream);
+}
+
+// ---- host drop-in (SPML packing, synchronous)
+enum HostOp { kXyaWa, kWaXya, kGradLon, kGradLat };
+
+static int host_call(sht_plan *h, const double *in, double *out, int nlev, HostOp op,
+                     const char *who) {
+  int rc = check_call(h, in, out, nlev, who);
+  if (rc) return rc;
+  Plan &p = h->p;
+  if (op == kGradLat && !(p.flags & SHT_FLAG_GRAD))
+    return fail(SHT_ERR_INVALID, std::string(who) + ": needs a plan built with SHT_FLAG_GRAD");
+  rc = ensure_host_staging(p);
+  if (rc) return rc;
+  hipStream_t s = p.host_stream;
+  const size_t ngrid = (size_t)nlev * p.jm * p.im;
+  const size_t nwa = (size_t)nlev * (p.N + 1) * (p.N + 1);
+  const size_t nin = (op == kWaXya) ? ngrid : nwa;
+  const size_t nout = (op == kWaXya) ? nwa : ngrid;
+  std::memcpy(p.pinned, in, sizeof(double) * nin);
+  SHT_HIP(hipMemcpyAsync(p.stage_a, p.pinned, sizeof(double) * nin, hipMemcpyHostToDevice, s));
+  switch (op) {
+    case kXyaWa:
+      if ((rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s))) return rc;
+      if ((rc = spec_to_grid(p, p.stage_b, p.stage_a, nlev, s))) return rc;
+      break;
+    case kGradLon:
+      if ((rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s))) return rc;
+      if ((rc = spec_to_grad(p, p.stage_b, nullptr, p.stage_a, nullptr, nlev, s))) return rc;
+      break;
+    case kGradLat:
+      if ((rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s))) return rc;
+      if ((rc = spec_to_grad(p, p.stage_b, nullptr, nullptr, p.stage_a, nlev, s))) return rc;
+      break;
+    case kWaXya:
+      if ((rc = grid_to_spec(p, p.stage_a, p.stage_b, nlev, s))) return rc;
+      if ((rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s))) return rc;
+      break;
+  }
+  SHT_HIP(hipMemcpyAsync(p.pinned, p.stage_a, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
+  SHT_HIP(hipStreamSynchronize(s));
+  std::memcpy(out, p.pinned, sizeof(double) * nout);
+  return SHT_OK;
+}
+
+int sht_xya_wa(sht_plan *h, const double *wa, double *xya, int nlev) {
+  return host_call(h, wa, xya, nlev, kXyaWa, "sht_xya_wa");
+}
+int sht_wa_xya(sht_plan *h, const double *xya, double *wa, int nlev) {
+  return host_call(h, xya, wa, nlev, kWaXya, "sht_wa_xya");
+}
+int sht_xya_gradlon_wa(sht_plan *h, const double *wa, double *xya, int nlev) {
+  return host_call(h, wa, xya, nlev, kGradLon, "sht_xya_gradlon_wa");
+}
+int sht_xya_gradlat_wa(sht_plan *h, const double *wa, double *xya, int nlev) {
+  return host_call(h, wa, xya, nlev, kGradLat, "sht_xya_gradlat_wa");
+}
+
+}  // extern "C"

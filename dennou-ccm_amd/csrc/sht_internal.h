@@ -1,0 +1,123 @@
+// sht_internal.h -- plan layout, error plumbing and kernel launchers shared by
+// the translation units of libsht_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+// The public entry points are the only exported symbols (-fvisibility=hidden).
+#pragma GCC visibility push(default)
+#include "../../include/sht.h"
+#pragma GCC visibility pop
+
+namespace sht {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define SHT_HIP(call)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return ::sht::fail(SHT_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ layout
+// Device tables of one plan.  P2 = jm/2 latitude pairs; pair p couples the
+// north node jn = P2 + p and the south node js = P2 - 1 - p (mu_js = -mu_jn).
+// Table rows over pairs are padded to P2p = roundup(P2, 16) with zeros.
+// Within one m the degrees split by parity of n - m: k-th even row n = m + 2k,
+// k-th odd row n = m + 2k + 1.
+//
+//   Pinv  per m: [par 2][KP(m)][P2p]  Pt_n^m(mu_p)          (inverse, p fastest)
+//   Hinv  per m: [par 2][KP(m)][P2p]  (1-mu^2) dPt/dmu      (inverse GradLat)
+//   Pfwd  per m: [par 2][P2p][KR(m)]  (w_p/2) Pt_n^m(mu_p)  (forward, k fastest)
+//   KP(m) = roundup(ceil((N-m+1)/2), 4), KR(m) = roundup(ceil((N-m+1)/2), 16),
+//   padding entries are zero.
+//
+//   four  [P2][N+1][par 2][ncol]  the Fourier intermediate: per latitude pair and
+//         wavenumber the equatorially even/odd halves, ncol = 2*nfield doubles
+//         (complex, field innermost).  Forward: (F(mu)+F(-mu), F(mu)-F(-mu));
+//         inverse: the even-n and odd-n Legendre sums (north = even + odd,
+//         south = even - odd).
+//
+// Mixed-precision tables (precision != FP64) hold the same Pinv/Pfwd entries
+// split into hi/lo 16-bit pieces, one scale exponent per m:
+//   Linv  per m: [par 2][KP16(m)][P2] x {hi, lo} packed as 16-bit pairs
+//   Lfwd  per m: [par 2][P2/32*32][KR(m)] ...
+// (see sht_legendre_mixed.hip).
+
+struct Plan {
+  int N = 0, im = 0, jm = 0, P2 = 0, P2p = 0, K = 0, kmax = 0, prec = 0, flags = 0;
+  int device = 0;
+  int maxke = 0;            // ceil((N+1)/2): most rows of one parity
+  // device tables
+  double *mu = nullptr, *w = nullptr, *coslat = nullptr; // jm each, S->N
+  double *twid = nullptr;                                // im complex: e^{2 pi i t/im}
+  double *Pinv = nullptr, *Hinv = nullptr, *Pfwd = nullptr;
+  int *poff_inv = nullptr, *poff_fwd = nullptr;          // N+1 offsets (doubles)
+  std::vector<int> h_poff_inv, h_poff_fwd;
+  size_t n_inv = 0, n_fwd = 0;
+  // mixed-precision tables
+  void *Linv = nullptr, *Lfwd = nullptr;
+  int *loff_inv = nullptr, *loff_fwd = nullptr;
+  size_t n_linv = 0, n_lfwd = 0;
+  // scratch
+  double *four = nullptr, *four2 = nullptr; // [P2][N+1][2][2*kmax]
+  double *stage_a = nullptr, *stage_b = nullptr; // device staging for host calls
+  double *pinned = nullptr;                      // pinned host staging
+  size_t pinned_bytes = 0;
+  hipStream_t host_stream = nullptr;
+  // host copies of the grid
+  std::vector<double> h_mu, h_w, h_coslat;
+};
+
+__host__ __device__ inline int roundup(int a, int b) { return (a + b - 1) / b * b; }
+__host__ __device__ inline int ceil_half(int a) { return (a + 1) / 2; }
+__host__ __device__ inline int64_t coef_off(int N, int m) {
+  return (int64_t)m * (N + 1) - (int64_t)m * (m - 1) / 2;
+}
+
+// ------------------------------------------------------------------ launchers
+// Table construction (sht_tables.hip)
+int build_tables(Plan &p);
+
+// FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos)
+int launch_fft_fwd(const Plan &p, const double *grid, double *four, int nf, hipStream_t s);
+int launch_fft_inv(const Plan &p, const double *four, const double *four2, double *grid,
+                   double *gradlon, double *gradlat, int nf, hipStream_t s);
+
+// Legendre stage, FP64 MFMA (sht_legendre.hip)
+int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
+                       bool value_table, bool deriv_table, int nf, hipStream_t s);
+int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s);
+
+// Legendre stage, mixed precision (sht_legendre_mixed.hip)
+int build_mixed_tables(Plan &p);
+int launch_leg_inv_mixed(const Plan &p, const double *spec, double *four, int nf, hipStream_t s);
+int launch_leg_fwd_mixed(const Plan &p, const double *four, double *spec, int nf, hipStream_t s);
+
+// Layout helpers (sht_pack.hip)
+int launch_spml_to_canon(const Plan &p, const double *wa, double *spec, int nf, hipStream_t s);
+int launch_canon_to_spml(const Plan &p, const double *spec, double *wa, int nf, hipStream_t s);
+int launch_avr_lonlat(const Plan &p, const double *grid, double *avg, int nf, hipStream_t s);
+
+// XCD-aware block order: the hardware deals consecutive workgroup ids to the 8
+// XCDs in turn; this maps them so that each XCD receives a contiguous range of
+// logical tiles (neighbouring tiles share table rows in that XCD's L2).
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+  const int nx = 8;
+  int xcd = b % nx, idx = b / nx;
+  int per = nblocks / nx, rem = nblocks % nx;
+  return xcd * per + (xcd < rem ? xcd : rem) + idx;
+}
+
+}  // namespace sht
+
+struct sht_plan {
+  sht::Plan p;
+};

@@ -1,0 +1,174 @@
+// sht_tables.hip -- plan-time FP64 kernels: Gauss-Legendre nodes and weights,
+// the normalised associated Legendre table Pt_n^m(mu_p), its latitude-derivative
+// table, and FFT twiddles.  All of it stays FP64 whatever the precision of the
+// Legendre GEMMs (SPML wa_Initial's work, run inside DynamicsHSplVAS83Init at
+// atm/dcpam_main_mod.f90:5036).
+#include "sht_internal.h"
+
+namespace sht {
+namespace {
+
+// Newton iteration on P_jm(cos theta) in the colatitude: cos(lat) = sin(theta)
+// then keeps full relative precision at the polar rings.
+__global__ void gauss_kernel(int jm, double *mu, double *w, double *coslat) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= jm / 2) return;
+  double th = M_PI * (4.0 * k + 3.0) / (4.0 * jm + 2.0);
+  double pn = 0.0, pn1 = 0.0;
+  for (int it = 0; it < 100; ++it) {
+    double s, x;
+    sincos(th, &s, &x);
+    double p0 = 1.0, p1 = x;
+    for (int n = 2; n <= jm; ++n) {
+      double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
+      p0 = p1;
+      p1 = p2;
+    }
+    pn = p1;
+    pn1 = p0;
+    double d = pn / (-jm * (pn1 - x * pn) / s);
+    th -= d;
+    if (fabs(d) < 1e-17 * (1.0 + th)) break;
+  }
+  double s, x;
+  sincos(th, &s, &x);
+  double p0 = 1.0, p1 = x;
+  for (int n = 2; n <= jm; ++n) {
+    double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
+    p0 = p1;
+    p1 = p2;
+  }
+  double dp = jm * (p0 - x * p1);
+  double wk = 2.0 * s * s / (dp * dp);
+  mu[jm - 1 - k] = x;
+  mu[k] = -x;
+  w[jm - 1 - k] = wk;
+  w[k] = wk;
+  coslat[jm - 1 - k] = s;
+  coslat[k] = s;
+}
+
+__device__ __forceinline__ double eps_nm(int n, int m) {
+  double nn = n, mm = m;
+  return sqrt((nn * nn - mm * mm) / (4.0 * nn * nn - 1.0));
+}
+
+// Extended exponent: a value is x * 2^(960 e), e <= 0 shared by the
+// recurrence's live terms.  The sectoral start sin^m(theta) falls below the
+// double range at high m near the poles (about 1e-345 at m = 170 on the T170
+// grid); the scaled form keeps the recurrence exact until the result is
+// representable again.
+__device__ __forceinline__ double xout(double x, int e) { return e == 0 ? x : ldexp(x, 960 * e); }
+
+// one thread per (latitude pair p, wavenumber m)
+__global__ void legendre_kernel(int N, int P2, int P2p, const double *__restrict__ mu,
+                                const double *__restrict__ coslat, const double *__restrict__ w,
+                                const int *__restrict__ poff_inv, const int *__restrict__ poff_fwd,
+                                double *__restrict__ Pinv, double *__restrict__ Pfwd,
+                                double *__restrict__ Hinv) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = t % P2, m = t / P2;
+  if (m > N) return;
+  const int jn = P2 + p; // north node of the pair
+  const double x = mu[jn], s = coslat[jn], hw = 0.5 * w[jn];
+  const int ke = ceil_half(N - m + 1);
+  const int KP = roundup(ke, 4), KR = roundup(ke, 16);
+  double *Pi = Pinv + poff_inv[m];
+  double *Pf = Pfwd + poff_fwd[m];
+  double *Hi = Hinv ? Hinv + poff_inv[m] : nullptr;
+
+  double pc = 1.0;
+  int e = 0;
+  for (int k = 1; k <= m; ++k) {
+    pc *= sqrt((2.0 * k + 1.0) / (2.0 * k)) * s;
+    if (pc != 0.0 && fabs(pc) < 0x1p-480) {
+      pc *= 0x1p960;
+      e -= 1;
+    }
+  }
+  double pp = 0.0; // Pt_{n-1}
+  for (int n = m; n <= N; ++n) {
+    // Pt_{n+1} (needed by the derivative of degree n)
+    double back = (n > m) ? eps_nm(n, m) * pp : 0.0;
+    double pn = (x * pc - back) / eps_nm(n + 1, m);
+    if (e < 0 && fabs(pn) >= 0x1p480) {
+      pn *= 0x1p-960;
+      pc *= 0x1p-960;
+      pp *= 0x1p-960;
+      e += 1;
+    }
+    const int d = n - m, par = d & 1, k = d >> 1;
+    const double v = xout(pc, e);
+    Pi[(size_t)(par * KP + k) * P2p + p] = v;
+    Pf[(size_t)(par * P2p + p) * KR + k] = hw * v;
+    if (Hi) {
+      // (1-mu^2) dPt_n/dmu = (n+1) eps_n Pt_{n-1} - n eps_{n+1} Pt_{n+1}
+      double h = (n > m ? (n + 1.0) * eps_nm(n, m) * pp : 0.0) - n * eps_nm(n + 1, m) * pn;
+      Hi[(size_t)(par * KP + k) * P2p + p] = xout(h, e);
+    }
+    pp = pc;
+    pc = pn;
+  }
+}
+
+__global__ void twiddle_kernel(int L, double *tw) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= L) return;
+  double s, c;
+  sincospi(2.0 * (double)t / (double)L, &s, &c);
+  tw[2 * t] = c;
+  tw[2 * t + 1] = s;
+}
+
+}  // namespace
+
+int build_tables(Plan &p) {
+  const int N = p.N, P2 = p.P2, P2p = p.P2p;
+  p.h_poff_inv.resize(N + 1);
+  p.h_poff_fwd.resize(N + 1);
+  size_t ti = 0, tf = 0;
+  for (int m = 0; m <= N; ++m) {
+    int ke = ceil_half(N - m + 1);
+    p.h_poff_inv[m] = (int)ti;
+    p.h_poff_fwd[m] = (int)tf;
+    ti += (size_t)2 * roundup(ke, 4) * P2p;
+    tf += (size_t)2 * P2p * roundup(ke, 16);
+  }
+  if (ti > (size_t)INT32_MAX || tf > (size_t)INT32_MAX)
+    return fail(SHT_ERR_UNSUPPORTED, "Legendre table exceeds 2^31 entries");
+  p.n_inv = ti;
+  p.n_fwd = tf;
+  SHT_HIP(hipMalloc(&p.mu, sizeof(double) * p.jm));
+  SHT_HIP(hipMalloc(&p.w, sizeof(double) * p.jm));
+  SHT_HIP(hipMalloc(&p.coslat, sizeof(double) * p.jm));
+  SHT_HIP(hipMalloc(&p.twid, sizeof(double) * 2 * p.im));
+  SHT_HIP(hipMalloc(&p.Pinv, sizeof(double) * ti));
+  SHT_HIP(hipMalloc(&p.Pfwd, sizeof(double) * tf));
+  if (p.flags & SHT_FLAG_GRAD) SHT_HIP(hipMalloc(&p.Hinv, sizeof(double) * ti));
+  SHT_HIP(hipMalloc(&p.poff_inv, sizeof(int) * (N + 1)));
+  SHT_HIP(hipMalloc(&p.poff_fwd, sizeof(int) * (N + 1)));
+  SHT_HIP(hipMemcpy(p.poff_inv, p.h_poff_inv.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+  SHT_HIP(hipMemcpy(p.poff_fwd, p.h_poff_fwd.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+  SHT_HIP(hipMemset(p.Pinv, 0, sizeof(double) * ti));
+  SHT_HIP(hipMemset(p.Pfwd, 0, sizeof(double) * tf));
+  if (p.Hinv) SHT_HIP(hipMemset(p.Hinv, 0, sizeof(double) * ti));
+
+  gauss_kernel<<<(P2 + 63) / 64, 64>>>(p.jm, p.mu, p.w, p.coslat);
+  SHT_HIP(hipGetLastError());
+  int nthr = P2 * (N + 1);
+  legendre_kernel<<<(nthr + 127) / 128, 128>>>(N, P2, P2p, p.mu, p.coslat, p.w, p.poff_inv,
+                                                 p.poff_fwd, p.Pinv, p.Pfwd, p.Hinv);
+  SHT_HIP(hipGetLastError());
+  twiddle_kernel<<<(p.im + 127) / 128, 128>>>(p.im, p.twid);
+  SHT_HIP(hipGetLastError());
+  SHT_HIP(hipDeviceSynchronize());
+  p.h_mu.resize(p.jm);
+  p.h_w.resize(p.jm);
+  p.h_coslat.resize(p.jm);
+  SHT_HIP(hipMemcpy(p.h_mu.data(), p.mu, sizeof(double) * p.jm, hipMemcpyDeviceToHost));
+  SHT_HIP(hipMemcpy(p.h_w.data(), p.w, sizeof(double) * p.jm, hipMemcpyDeviceToHost));
+  SHT_HIP(hipMemcpy(p.h_coslat.data(), p.coslat, sizeof(double) * p.jm, hipMemcpyDeviceToHost));
+  return SHT_OK;
+}
+
+}  // namespace sht

@@ -1,0 +1,194 @@
+"""Python binding of libsht_hip.so (include/sht.h) through ctypes.
+
+The library is the product; this module only loads it and passes pointers.
+Device arrays are torch tensors (device memory and streams are torch's
+plumbing); the transform work is the library's HIP kernels.  There is no
+fallback: if the library is missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(os.path.dirname(_HERE))  # dennou-ccm_amd/
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libsht_hip.so")
+HEADER = os.path.join(os.path.dirname(PKG_ROOT), "include", "sht.h")
+
+SHT_OK = 0
+SHT_ERR_INVALID = -1
+SHT_ERR_NOMEM = -2
+SHT_ERR_DEVICE = -3
+SHT_ERR_UNSUPPORTED = -4
+SHT_ERR_BUSY = -5
+
+PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16 = 0, 1, 2
+FLAG_GRAD = 1
+
+_lib = None
+
+
+class ShtError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class _Desc(ctypes.Structure):
+    _fields_ = [("nmax", ctypes.c_int), ("im", ctypes.c_int), ("jm", ctypes.c_int),
+                ("kmax", ctypes.c_int), ("precision", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+_vp = ctypes.c_void_p
+_ci = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); mirrors include/sht.h
+SIGNATURES = {
+    "sht_plan_create": (_ci, [ctypes.POINTER(_Desc), ctypes.POINTER(_vp)]),
+    "sht_plan_destroy": (None, [_vp]),
+    "sht_last_error": (ctypes.c_char_p, []),
+    "sht_abi_version": (_ci, []),
+    "sht_plan_grid": (_ci, [_vp, _dp, _dp, _dp, _dp]),
+    "sht_l_nm": (_ci, [_ci, _ci]),
+    "sht_nm_l": (_ci, [_ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
+    "sht_spec_to_grid_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_grid_to_spec_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_spec_to_grad_dev": (_ci, [_vp, _vp, _vp, _vp, _vp, _ci, _vp]),
+    "sht_spml_to_canon_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_canon_to_spml_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_avr_lonlat_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_xya_wa": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_wa_xya": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_xya_gradlon_wa": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_xya_gradlat_wa": (_ci, [_vp, _dp, _dp, _ci]),
+}
+
+
+def lib():
+    """Load libsht_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != SHT_OK:
+        raise ShtError(rc, lib().sht_last_error().decode())
+    return rc
+
+
+def ncoef(nmax: int) -> int:
+    return (nmax + 1) * (nmax + 2) // 2
+
+
+def _ptr(t) -> int | None:
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+def _stream_handle(stream) -> int | None:
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _host(a) -> "ctypes._Pointer":
+    return a.ctypes.data_as(_dp)
+
+
+class Plan:
+    """One transform plan (SPML wa_Initial).  Device calls take torch tensors:
+    spectra as float64 (K, nfield, 2) or (K*nfield*2,) in the canonical
+    layout, grids as float64 (nfield, jm, im)."""
+
+    def __init__(self, nmax: int, im: int, jm: int, kmax: int, precision: int = PREC_FP64,
+                 flags: int = 0):
+        d = _Desc(nmax, im, jm, kmax, precision, flags)
+        h = _vp()
+        check(lib().sht_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+        self.nmax, self.im, self.jm, self.kmax = nmax, im, jm, kmax
+        self.precision, self.flags = precision, flags
+        self.K = ncoef(nmax)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().sht_plan_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self):
+        lon, lat = np.empty(self.im), np.empty(self.jm)
+        lw, yw = np.empty(self.im), np.empty(self.jm)
+        check(lib().sht_plan_grid(self._h, _host(lon), _host(lat), _host(lw), _host(yw)))
+        return lon, lat, lw, yw
+
+    # ---- device (canonical spectra, stream ordered)
+    def spec_to_grid(self, spec, grid, nfield: int, stream=None):
+        check(lib().sht_spec_to_grid_dev(self._h, _ptr(spec), _ptr(grid), nfield, _stream_handle(stream)))
+
+    def grid_to_spec(self, grid, spec, nfield: int, stream=None):
+        check(lib().sht_grid_to_spec_dev(self._h, _ptr(grid), _ptr(spec), nfield, _stream_handle(stream)))
+
+    def spec_to_grad(self, spec, grid, gradlon, gradlat, nfield: int, stream=None):
+        check(lib().sht_spec_to_grad_dev(self._h, _ptr(spec), _ptr(grid), _ptr(gradlon), _ptr(gradlat),
+                                         nfield, _stream_handle(stream)))
+
+    def spml_to_canon(self, wa, spec, nfield: int, stream=None):
+        check(lib().sht_spml_to_canon_dev(self._h, _ptr(wa), _ptr(spec), nfield, _stream_handle(stream)))
+
+    def canon_to_spml(self, spec, wa, nfield: int, stream=None):
+        check(lib().sht_canon_to_spml_dev(self._h, _ptr(spec), _ptr(wa), nfield, _stream_handle(stream)))
+
+    def avr_lonlat(self, grid, avg, nfield: int, stream=None):
+        check(lib().sht_avr_lonlat_dev(self._h, _ptr(grid), _ptr(avg), nfield, _stream_handle(stream)))
+
+    # ---- host drop-in (SPML packing), numpy arrays
+    def _host_call(self, fn, inp, out_shape):
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        out = np.empty(out_shape)
+        check(fn(self._h, _host(inp), _host(out), inp.shape[0]))
+        return out
+
+    def xya_wa(self, wa):
+        return self._host_call(lib().sht_xya_wa, wa, (wa.shape[0], self.jm, self.im))
+
+    def wa_xya(self, xya):
+        return self._host_call(lib().sht_wa_xya, xya, (xya.shape[0], (self.nmax + 1) ** 2))
+
+    def xya_gradlon_wa(self, wa):
+        return self._host_call(lib().sht_xya_gradlon_wa, wa, (wa.shape[0], self.jm, self.im))
+
+    def xya_gradlat_wa(self, wa):
+        return self._host_call(lib().sht_xya_gradlat_wa, wa, (wa.shape[0], self.jm, self.im))
+
+
+def spec_to_real(spec_c: np.ndarray) -> np.ndarray:
+    """complex (K, nf) -> float64 (K, nf, 2) canonical device layout."""
+    return np.ascontiguousarray(np.stack([spec_c.real, spec_c.imag], axis=-1))
+
+
+def real_to_spec(a: np.ndarray) -> np.ndarray:
+    a = np.asarray(a)
+    return a[..., 0] + 1j * a[..., 1]

@@ -1,0 +1,130 @@
+/*
+ * sht.h -- C ABI of libsht_hip.so, the MI355X (gfx950) spherical-harmonic
+ * transform on a Gaussian grid with triangular truncation.
+ *
+ * It replaces SPML's wa_module (linked by Dennou-CCM as -lspml-omp -lisp-fort,
+ * Mkinclude:15-21 and :83-84), which DCPAM's dynamical core calls inside the
+ * 'Dyn' region of atm/dcpam_main_mod.f90:2167-2203 (DynamicsHSplVAS83 at
+ * :2173-2180, initialised at :5036).  SPML and ISPACK are not in the reference
+ * tree; each entry point below names the SPML routine it stands in for, and the
+ * in-tree call site or data layout that fixes its arguments.
+ *
+ * Conventions
+ *   grid     xya[f][j][i]   FP64, i = 0..im-1 longitude (2 pi i/im), j = 0..jm-1
+ *            Gauss latitudes south to north, f = level/field: the Fortran array
+ *            xya(0:im-1, 1:jm, 1:k) of atm/dcpam_main_mod.f90:4107-4125.
+ *   spectra, SPML packing (host drop-in):  wa[f][l], l = n(n+1)+m, m = -n..n,
+ *            (N+1)^2 reals per field: Fortran wa((N+1)**2, k).  Coefficients of
+ *            real harmonics with unit mean square over the sphere:
+ *            Pt_n^0, sqrt2 Pt_n^m cos(m lon), sqrt2 Pt_n^m sin(m lon) (m<0 slot).
+ *   spectra, canonical (device entry points): complex s_n^m,
+ *            spec[((off(m) + n - m) * nf + f) * 2 + {0,1}], off(m) = m(N+1) - m(m-1)/2,
+ *            f(lon,mu) = sum_n sum_{|m|<=n} s_n^m Pt_n^|m|(mu) e^{i m lon},
+ *            Pt normalised to (1/2) int Pt^2 dmu = 1 (the zonal normalisation of
+ *            exp/APEI07Couple/ebm/EBM_main.f90:170-174, :242-246).
+ *            wa(n,0) = Re s_n^0, wa(n,m) = sqrt2 Re s_n^m, wa(n,-m) = -sqrt2 Im s_n^m.
+ *   gradients  GradLon = (1/cos lat) df/dlon, GradLat = df/dlat (unit sphere).
+ *
+ * Errors: every int-returning call returns SHT_OK or a negative SHT_ERR_*; the
+ * text is in sht_last_error() (thread-local).  SPML aborts through
+ * MessageNotify instead (style at atm/dccm_atm_mod.f90:144-147); the Fortran
+ * module in dennou-ccm_amd/fortran turns a nonzero code into an error stop.
+ *
+ * Threading: a plan is immutable after creation apart from its scratch; calls
+ * on one plan are stream ordered and must not overlap on different streams.
+ * Different plans may run concurrently on different streams.
+ */
+#ifndef SHT_H
+#define SHT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHT_ABI_VERSION 1
+
+enum {
+  SHT_OK = 0,
+  SHT_ERR_INVALID = -1,     /* bad argument or plan descriptor */
+  SHT_ERR_NOMEM = -2,       /* device or host allocation failed */
+  SHT_ERR_DEVICE = -3,      /* a HIP call failed */
+  SHT_ERR_UNSUPPORTED = -4, /* valid but not implemented (e.g. im not a power of 2) */
+  SHT_ERR_BUSY = -5         /* request server: no memory or slot free for admission */
+};
+
+/* Arithmetic of the Legendre step.  The FFT, the tables and the final
+ * spectral accumulation are FP64 in every form. */
+enum {
+  SHT_PREC_FP64 = 0,        /* FP64 MFMA (v_mfma_f64_16x16x4) */
+  SHT_PREC_SPLIT_BF16 = 1,  /* BF16 hi/lo splits, three products, FP32 accumulate */
+  SHT_PREC_SCALED_FP16 = 2  /* per-m column-scaled FP16 hi/lo, FP32 accumulate */
+};
+
+enum { SHT_FLAG_GRAD = 1 }; /* also build the derivative table (GradLat) */
+
+typedef struct sht_plan sht_plan;
+
+typedef struct {
+  int nmax;      /* triangular truncation N (gridset_nml nmax, dcpam_APEI07Couple_T21L16.conf:56-66) */
+  int im;        /* longitudes (imax); a power of two, >= 3N+1 for alias-free products */
+  int jm;        /* Gaussian latitudes (jmax); even, > N */
+  int kmax;      /* largest number of fields per call (sizes the plan's scratch) */
+  int precision; /* SHT_PREC_* */
+  int flags;     /* SHT_FLAG_* */
+} sht_desc;
+
+/* ---- plan: SPML wa_Initial(n, i, j, k) / wa_Finalize (called inside
+ *      DynamicsHSplVAS83Init, atm/dcpam_main_mod.f90:5036).  Gauss nodes and
+ *      weights, Pt_n^m and derivative tables (FP64, computed on the GPU),
+ *      FFT twiddles.  Uses the current HIP device. */
+int sht_plan_create(const sht_desc *desc, sht_plan **plan);
+void sht_plan_destroy(sht_plan *plan);
+const char *sht_last_error(void);
+int sht_abi_version(void);
+
+/* ---- grid description: SPML w_module xy_Lon, xy_Lat, x_Lon_Weight,
+ *      y_Lat_Weight (read by tool/gmapgen/gmapgen_main.f90:296-303).
+ *      Host arrays of im, jm, im, jm doubles; any pointer may be NULL.
+ *      Latitudes in radians, S->N; latitude weights sum to 2. */
+int sht_plan_grid(const sht_plan *plan, double *lon, double *lat,
+                  double *lon_weight, double *lat_weight);
+
+/* ---- spectral index helpers: SPML l_nm / nm_l (commented use at
+ *      exp/APEI07Couple/ebm/EBM_main.f90:100, :122).  1-based l as in Fortran. */
+int sht_l_nm(int n, int m);
+int sht_nm_l(int l, int *n, int *m);
+
+/* ---- device entry points, canonical spectra, stream ordered (no host sync).
+ *      stream is a hipStream_t (NULL = legacy default stream).               */
+/* xya_wa: spectral -> grid */
+int sht_spec_to_grid_dev(sht_plan *plan, const double *spec, double *grid,
+                         int nfield, void *stream);
+/* wa_xya: grid -> spectral */
+int sht_grid_to_spec_dev(sht_plan *plan, const double *grid, double *spec,
+                         int nfield, void *stream);
+/* xya_wa, xya_GradLon_wa and xya_GradLat_wa of one spectral input in one pass;
+ * any of the three outputs may be NULL.  GradLat needs SHT_FLAG_GRAD. */
+int sht_spec_to_grad_dev(sht_plan *plan, const double *spec, double *grid,
+                         double *gradlon, double *gradlat, int nfield, void *stream);
+/* SPML real packing <-> canonical complex layout */
+int sht_spml_to_canon_dev(const sht_plan *plan, const double *wa, double *spec,
+                          int nfield, void *stream);
+int sht_canon_to_spml_dev(const sht_plan *plan, const double *spec, double *wa,
+                          int nfield, void *stream);
+/* AvrLonLat_xy: global mean of each field (atm/dccm_atm_mod.f90:668-669, :774) */
+int sht_avr_lonlat_dev(const sht_plan *plan, const double *grid, double *avg,
+                       int nfield, void *stream);
+
+/* ---- host drop-in, SPML packing, synchronous: wa_module's array-valued
+ *      functions.  nlev <= kmax.  Arrays are host memory.                    */
+int sht_xya_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
+int sht_wa_xya(sht_plan *plan, const double *xya, double *wa, int nlev);
+int sht_xya_gradlon_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
+int sht_xya_gradlat_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHT_H */
