@@ -1,0 +1,252 @@
+"""Benchmark of the spherical-harmonic transform on one MI355X.
+
+A step is one forward+inverse pass of the transform over one batch: T85L32
+(256 x 128 Gaussian grid, vorticity, divergence, temperature on 32 levels plus
+ln p_s = 97 fields), spectral -> grid -> spectral, with inputs resident in HBM.
+fields/s counts fields that finish the round trip.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|bf16x3|fp16x3|auto]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "dennou-ccm_amd", "py")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "T85L32 SHT fields/s, split-BF16 MFMA vs FP64 at <=1e-6 err; serve p50/p99 ms"
+N, IM, JM, NLEV = 85, 256, 128, 32
+NF = 3 * NLEV + 1
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+
+PRECISIONS = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
+SPIN_CYCLES = 200_000_000  # ~0.1 s: longer than it takes the host to queue K steps
+
+
+def algorithmic(plan_tables_bytes: dict, nf: int):
+    """Per-launch algorithmic bytes and flops of each stage (FP64 I/O)."""
+    K = (N + 1) * (N + 2) // 2
+    four = 16 * (N + 1) * JM        # Fourier intermediate per field
+    grid = 8 * IM * JM
+    spec = 16 * K
+    fft_flops = 2.5 * JM * IM * np.log2(IM)
+    leg_flops = 2.0 * JM * K
+    return {
+        "leg_inv": (nf * (spec + four) + plan_tables_bytes["inv"], nf * leg_flops),
+        "fft_inv": (nf * (four + grid), nf * fft_flops),
+        "fft_fwd": (nf * (grid + four), nf * fft_flops),
+        "leg_fwd": (nf * (four + spec) + plan_tables_bytes["fwd"], nf * leg_flops),
+    }
+
+
+def table_bytes(precision: int):
+    P2p = ((JM // 2) + 15) // 16 * 16
+    inv = fwd = 0
+    for m in range(N + 1):
+        ke = (N - m + 2) // 2
+        inv += 2 * ((ke + 3) // 4 * 4) * P2p
+        fwd += 2 * P2p * ((ke + 15) // 16 * 16)
+    esz = 8 if precision == 0 else 4  # hi/lo 16-bit pairs
+    return {"inv": inv * esz, "fwd": fwd * esz}
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+def run_gpu(precision: int, steps: int, warmup: int):
+    import torch
+
+    import shtamd
+    from shtamd import synth
+
+    dev = torch.device("cuda:0")
+    spec_c = synth.state_spectra(N, NLEV)
+    plan = shtamd.Plan(N, IM, JM, NF, precision=precision)
+    s_in = torch.from_numpy(shtamd.spec_to_real(spec_c)).to(dev)
+    grid = torch.empty((NF, JM, IM), dtype=torch.float64, device=dev)
+    s_out = torch.empty_like(s_in)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        plan.spec_to_grid(s_in, grid, NF, stream)
+        plan.grid_to_spec(grid, s_out, NF, stream)
+
+    for _ in range(max(warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    out_grid = grid.cpu().numpy().copy()
+    out_spec = shtamd.real_to_spec(s_out.cpu().numpy())
+
+    # Each timed loop starts behind a bounded spin kernel (torch.cuda._sleep), so
+    # the host has queued all K steps before the GPU reaches them: the events
+    # then bracket back-to-back GPU execution, not host launch latency.
+    def prefill():
+        torch.cuda._sleep(SPIN_CYCLES)
+
+    # eager, K steps bracketed by events on the launch stream
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    prefill()
+    e0.record(stream)
+    for _ in range(steps):
+        step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    eager_ms = e0.elapsed_time(e1) / steps
+
+    # the same step captured once as a HIP graph and replayed K times
+    graph_ms = None
+    try:
+        g = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream()
+        cap.wait_stream(stream)
+        with torch.cuda.stream(cap):
+            step()  # warm the capture stream
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g, stream=cap):
+            plan.spec_to_grid(s_in, grid, NF, cap)
+            plan.grid_to_spec(grid, s_out, NF, cap)
+        for _ in range(max(warmup, 1)):
+            g.replay()
+        torch.cuda.synchronize()
+        if not torch.equal(s_out.cpu(), torch.from_numpy(shtamd.spec_to_real(out_spec))):
+            raise RuntimeError("graph replay differs from eager")
+        # replay() launches on the current stream: bracket it there
+        g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        prefill()
+        g0.record(stream)
+        for _ in range(steps):
+            g.replay()
+        g1.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = g0.elapsed_time(g1) / steps
+    except Exception as exc:  # graph capture is an optimisation; report why it was skipped
+        print(f"# graph replay skipped: {exc}", file=sys.stderr)
+
+    # per-kernel durations: the library's own event timers around each launch
+    plan.set_timing(True)
+    prefill()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    st = plan.stage_times()
+    plan.set_timing(False)
+    kernel_ms = {k: (v[0] / v[1] if v[1] else None) for k, v in st.items()}
+    plan.close()
+    return {"eager_ms": eager_ms, "graph_ms": graph_ms, "kernel_ms": kernel_ms,
+            "grid": out_grid, "spec": out_spec, "spec_in": spec_c}
+
+
+def cpu_baseline(spec_c, seconds: float, threads: int):
+    import oracle
+
+    p = oracle.Plan(N, IM, JM)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        g = p.spec_to_grid(spec_c, nthreads=threads)
+        p.grid_to_spec(g, nthreads=threads)
+        n += 1
+        dt = time.perf_counter() - t0
+        if dt >= seconds and n >= 3:
+            break
+    return n * NF / dt, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", default="fp64", choices=sorted(PRECISIONS) + ["auto"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+                                   "(the transform shards by field batch; see DESIGN.md)"}))
+        sys.exit(2)
+
+    import oracle
+
+    prec_name = "fp64" if args.precision == "auto" else args.precision
+    prec = PRECISIONS[prec_name]
+    res = run_gpu(prec, args.steps, args.warmup)
+    fp64 = res if prec == 0 else run_gpu(0, args.steps, args.warmup)
+
+    # error of this run's outputs against the FP64 oracle on the same inputs
+    ref = oracle.Plan(N, IM, JM)
+    g_ref = ref.spec_to_grid(res["spec_in"])
+    err_grid = rel(res["grid"], g_ref)
+    err_spec = rel(res["spec"], ref.grid_to_spec(res["grid"]))
+    err_rt = rel(res["spec"], res["spec_in"])
+
+    best_ms = min(x for x in (res["eager_ms"], res["graph_ms"]) if x is not None)
+    value = NF / (best_ms * 1e-3)
+
+    # roofline of the dominant kernel
+    tb = table_bytes(prec)
+    alg = algorithmic(tb, NF)
+    kms = {k: v for k, v in res["kernel_ms"].items() if v}
+    dom = max(kms, key=kms.get)
+    bytes_, flops = alg[dom]
+    achieved = bytes_ / (kms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "kernel_us": round(kms[dom] * 1e3, 2), "algorithmic_bytes": int(bytes_)}
+    kernels = {}
+    for k, ms in kms.items():
+        b, f = alg[k]
+        kernels[k] = {"us": round(ms * 1e3, 2), "GB/s": round(b / (ms * 1e-3) / 1e9, 1),
+                      "TFLOP/s": round(f / (ms * 1e-3) / 1e12, 3)}
+    for k in ("leg_inv", "leg_fwd"):
+        if k in kernels:
+            peak = FP64_PEAK_TFLOPS if prec == 0 else BF16_PEAK_TFLOPS
+            kernels[k]["mfma_frac"] = round(kernels[k]["TFLOP/s"] / peak, 4)
+
+    threads = min(16, os.cpu_count() or 1)
+    cpu_v, cpu_n, cpu_dt = cpu_baseline(res["spec_in"], args.cpu_seconds, threads)
+    cpu1_v, cpu1_n, cpu1_dt = cpu_baseline(res["spec_in"], max(2.0, args.cpu_seconds / 4), 1)
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "fields/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best_ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {0: "f64", 1: "bf16x3/f32acc/f64-final", 2: "f16x3/f32acc/f64-final"}[prec],
+        "data": "synthetic (seeded CN(0,n^-3) spectra, master seed 20261202)",
+        "config": {"workload": "T85L32 forward+inverse SHT: 97 fields, 256x128 Gaussian grid, N=85",
+                   "fields_per_step": NF, "precision": prec_name, "timing": "graph" if
+                   res["graph_ms"] is not None and res["graph_ms"] <= res["eager_ms"] else "eager"},
+        "eager_ms_per_step": round(res["eager_ms"], 5),
+        "graph_ms_per_step": None if res["graph_ms"] is None else round(res["graph_ms"], 5),
+        "fp64_fields_per_s": round(NF / (min(x for x in (fp64["eager_ms"], fp64["graph_ms"]) if x) * 1e-3), 1),
+        "err_vs_fp64_oracle": {"grid_rel_l2": err_grid, "spec_rel_l2": err_spec, "roundtrip_rel_l2": err_rt},
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": {"value": round(cpu_v, 1), "unit": "fields/s", "cores": threads, "kind": "port",
+                         "sample": f"oracle round trips of the same 97-field T85L32 batch, {cpu_n} reps "
+                                   f"in {cpu_dt:.1f} s, {threads} OpenMP threads",
+                         "single_core_value": round(cpu1_v, 1)},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -39,6 +39,11 @@ static int validate_desc(const sht_desc *d) {
 }
 
 static void release(Plan &p) {
+  for (auto &t : p.timed) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (hipEvent_t e : p.event_pool) (void)hipEventDestroy(e);
   void *ptrs[] = {p.mu, p.w, p.coslat, p.twid, p.Pinv, p.Hinv, p.Pfwd, p.poff_inv, p.poff_fwd,
                   p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.four, p.four2, p.stage_a, p.stage_b};
   for (void *q : ptrs)
@@ -70,21 +75,48 @@ static int ensure_host_staging(Plan &p) {
   return SHT_OK;
 }
 
+static hipEvent_t pooled_event(Plan &p) {
+  if (!p.event_pool.empty()) {
+    hipEvent_t e = p.event_pool.back();
+    p.event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Runs one launcher; with timing on, brackets it with events on its stream.
+template <class F>
+static int timed(Plan &p, int stage, hipStream_t s, F &&launch) {
+  if (!p.timing) return launch();
+  hipEvent_t a = pooled_event(p), b = pooled_event(p);
+  if (!a || !b) return fail(SHT_ERR_DEVICE, "stage timer: hipEventCreate failed");
+  SHT_HIP(hipEventRecord(a, s));
+  int rc = launch();
+  SHT_HIP(hipEventRecord(b, s));
+  p.timed.push_back({stage, a, b});
+  return rc;
+}
+
 static int spec_to_grid(Plan &p, const double *spec, double *grid, int nf, hipStream_t s) {
-  int rc;
-  if (p.prec == SHT_PREC_FP64)
-    rc = launch_leg_inv_f64(p, spec, p.four, p.four2, true, false, nf, s);
-  else
-    rc = launch_leg_inv_mixed(p, spec, p.four, nf, s);
+  int rc = timed(p, SHT_STAGE_LEG_INV, s, [&] {
+    return p.prec == SHT_PREC_FP64 ? launch_leg_inv_f64(p, spec, p.four, p.four2, true, false, nf, s)
+                                   : launch_leg_inv_mixed(p, spec, p.four, nf, s);
+  });
   if (rc) return rc;
-  return launch_fft_inv(p, p.four, nullptr, grid, nullptr, nullptr, nf, s);
+  return timed(p, SHT_STAGE_FFT_INV, s, [&] {
+    return launch_fft_inv(p, p.four, nullptr, grid, nullptr, nullptr, nf, s);
+  });
 }
 
 static int grid_to_spec(Plan &p, const double *grid, double *spec, int nf, hipStream_t s) {
-  int rc = launch_fft_fwd(p, grid, p.four, nf, s);
+  int rc = timed(p, SHT_STAGE_FFT_FWD, s, [&] { return launch_fft_fwd(p, grid, p.four, nf, s); });
   if (rc) return rc;
-  if (p.prec == SHT_PREC_FP64) return launch_leg_fwd_f64(p, p.four, spec, nf, s);
-  return launch_leg_fwd_mixed(p, p.four, spec, nf, s);
+  return timed(p, SHT_STAGE_LEG_FWD, s, [&] {
+    return p.prec == SHT_PREC_FP64 ? launch_leg_fwd_f64(p, p.four, spec, nf, s)
+                                   : launch_leg_fwd_mixed(p, p.four, spec, nf, s);
+  });
 }
 
 // Gradients always run the FP64 Legendre step (the derivative table is FP64 only).
@@ -92,9 +124,13 @@ static int spec_to_grad(Plan &p, const double *spec, double *grid, double *glon,
                         int nf, hipStream_t s) {
   const bool value_tab = grid || glon;
   const bool deriv_tab = glat != nullptr;
-  int rc = launch_leg_inv_f64(p, spec, p.four, p.four2, value_tab, deriv_tab, nf, s);
+  int rc = timed(p, SHT_STAGE_LEG_INV, s, [&] {
+    return launch_leg_inv_f64(p, spec, p.four, p.four2, value_tab, deriv_tab, nf, s);
+  });
   if (rc) return rc;
-  return launch_fft_inv(p, p.four, p.four2, grid, glon, glat, nf, s);
+  return timed(p, SHT_STAGE_FFT_INV, s, [&] {
+    return launch_fft_inv(p, p.four, p.four2, grid, glon, glat, nf, s);
+  });
 }
 
 }  // namespace sht
@@ -163,10 +199,43 @@ int sht_plan_grid(const sht_plan *h, double *lon, double *lat, double *lon_weigh
     if (lon_weight) lon_weight[i] = 2.0 * M_PI / (double)p.im;
   }
   for (int j = 0; j < p.jm; ++j) {
-    if (lat) lat[j] = (p.h_mu[j] < 0 ? -1.0 : 1.0) * acos(p.h_coslat[j] > 1.0 ? 1.0 : p.h_coslat[j]);
+    if (lat) lat[j] = atan2(p.h_mu[j], p.h_coslat[j]);
     if (lat_weight) lat_weight[j] = p.h_w[j];
   }
   return SHT_OK;
+}
+
+int sht_plan_set_timing(sht_plan *h, int enable) {
+  if (!h) return fail(SHT_ERR_INVALID, "sht_plan_set_timing: plan is NULL");
+  h->p.timing = enable != 0;
+  return SHT_OK;
+}
+
+int sht_plan_stage_times(sht_plan *h, double *total_ms, long long *launches) {
+  if (!h) return fail(SHT_ERR_INVALID, "sht_plan_stage_times: plan is NULL");
+  Plan &p = h->p;
+  double ms[SHT_NSTAGE] = {0, 0, 0, 0};
+  long long cnt[SHT_NSTAGE] = {0, 0, 0, 0};
+  int rc = SHT_OK;
+  for (auto &t : p.timed) {
+    float e = 0.f;
+    if (rc == SHT_OK) {
+      hipError_t st = hipEventSynchronize(t.b);
+      if (st == hipSuccess) st = hipEventElapsedTime(&e, t.a, t.b);
+      if (st != hipSuccess)
+        rc = fail(SHT_ERR_DEVICE, std::string("sht_plan_stage_times: ") + hipGetErrorString(st));
+    }
+    ms[t.stage] += e;
+    cnt[t.stage] += 1;
+    p.event_pool.push_back(t.a);
+    p.event_pool.push_back(t.b);
+  }
+  p.timed.clear();
+  for (int i = 0; i < SHT_NSTAGE; ++i) {
+    if (total_ms) total_ms[i] = ms[i];
+    if (launches) launches[i] = cnt[i];
+  }
+  return rc;
 }
 
 int sht_l_nm(int n, int m) {

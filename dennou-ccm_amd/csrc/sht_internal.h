@@ -74,6 +74,11 @@ struct Plan {
   hipStream_t host_stream = nullptr;
   // host copies of the grid
   std::vector<double> h_mu, h_w, h_coslat;
+  // stage timers (sht_plan_set_timing)
+  bool timing = false;
+  struct TimedLaunch { int stage; hipEvent_t a, b; };
+  std::vector<TimedLaunch> timed;
+  std::vector<hipEvent_t> event_pool;
 };
 
 __host__ __device__ inline int roundup(int a, int b) { return (a + b - 1) / b * b; }

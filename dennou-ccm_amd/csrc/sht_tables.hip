@@ -8,38 +8,54 @@
 namespace sht {
 namespace {
 
-// Newton iteration on P_jm(cos theta) in the colatitude: cos(lat) = sin(theta)
-// then keeps full relative precision at the polar rings.
+// P_n(cos theta) and (1 - x^2) dP_n/dx = n (P_{n-1} - x P_n).  Near the pole
+// (x > 1/2) the Bonnet recurrence runs in its difference form on
+// u = 1 - x = 2 sin^2(theta/2) taken from theta, so the node's colatitude is
+// resolved to its own rounding instead of eps/sin(theta):
+//   D_k = ((k-1) D_{k-1} - (2k-1) u P_{k-1}) / k,  P_k = P_{k-1} + D_k,
+//   P_{n-1} - x P_n = u P_n - D_n.
+__device__ void legendre_pn_theta(int n, double th, double *pn, double *dq) {
+  double x = cos(th);
+  if (x > 0.5) {
+    double sh = sin(0.5 * th), u = 2.0 * sh * sh;
+    double p = 1.0, d = 0.0;
+    for (int k = 1; k <= n; ++k) {
+      d = ((k - 1.0) * d - (2.0 * k - 1.0) * u * p) / k;
+      p += d;
+    }
+    *pn = p;
+    *dq = n * (u * p - d);
+  } else {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    *pn = p1;
+    *dq = n * (p0 - x * p1);
+  }
+}
+
+// Newton iteration in the colatitude, one thread per northern root.
 __global__ void gauss_kernel(int jm, double *mu, double *w, double *coslat) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= jm / 2) return;
   double th = M_PI * (4.0 * k + 3.0) / (4.0 * jm + 2.0);
-  double pn = 0.0, pn1 = 0.0;
+  double pn = 0.0, dq = 0.0;
   for (int it = 0; it < 100; ++it) {
-    double s, x;
-    sincos(th, &s, &x);
-    double p0 = 1.0, p1 = x;
-    for (int n = 2; n <= jm; ++n) {
-      double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
-      p0 = p1;
-      p1 = p2;
-    }
-    pn = p1;
-    pn1 = p0;
-    double d = pn / (-jm * (pn1 - x * pn) / s);
+    legendre_pn_theta(jm, th, &pn, &dq);
+    double d = -pn * sin(th) / dq;
     th -= d;
-    if (fabs(d) < 1e-17 * (1.0 + th)) break;
+    if (fabs(d) < 1e-15 * th) {  // converged: one more step, then stop
+      legendre_pn_theta(jm, th, &pn, &dq);
+      th -= -pn * sin(th) / dq;
+      break;
+    }
   }
-  double s, x;
-  sincos(th, &s, &x);
-  double p0 = 1.0, p1 = x;
-  for (int n = 2; n <= jm; ++n) {
-    double p2 = ((2.0 * n - 1.0) * x * p1 - (n - 1.0) * p0) / n;
-    p0 = p1;
-    p1 = p2;
-  }
-  double dp = jm * (p0 - x * p1);
-  double wk = 2.0 * s * s / (dp * dp);
+  legendre_pn_theta(jm, th, &pn, &dq);
+  double s = sin(th), x = cos(th);
+  double wk = 2.0 * s * s / (dq * dq);
   mu[jm - 1 - k] = x;
   mu[k] = -x;
   w[jm - 1 - k] = wk;

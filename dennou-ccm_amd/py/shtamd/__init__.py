@@ -64,7 +64,11 @@ SIGNATURES = {
     "sht_wa_xya": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_xya_gradlon_wa": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_xya_gradlat_wa": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_plan_set_timing": (_ci, [_vp, _ci]),
+    "sht_plan_stage_times": (_ci, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 }
+
+STAGES = ("leg_inv", "fft_inv", "fft_fwd", "leg_fwd")
 
 
 def lib():
@@ -163,6 +167,17 @@ class Plan:
 
     def avr_lonlat(self, grid, avg, nfield: int, stream=None):
         check(lib().sht_avr_lonlat_dev(self._h, _ptr(grid), _ptr(avg), nfield, _stream_handle(stream)))
+
+    # ---- stage timers
+    def set_timing(self, enable: bool):
+        check(lib().sht_plan_set_timing(self._h, 1 if enable else 0))
+
+    def stage_times(self):
+        """{stage: (total_ms, launches)} since the last call (waits for the GPU)."""
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_longlong * 4)()
+        check(lib().sht_plan_stage_times(self._h, ms, n))
+        return {STAGES[i]: (ms[i], n[i]) for i in range(4)}
 
     # ---- host drop-in (SPML packing), numpy arrays
     def _host_call(self, fn, inp, out_shape):

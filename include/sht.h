@@ -124,6 +124,22 @@ int sht_wa_xya(sht_plan *plan, const double *xya, double *wa, int nlev);
 int sht_xya_gradlon_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
 int sht_xya_gradlat_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
 
+/* ---- stage timers: HIP events around every kernel launch of a plan, summed
+ *      per stage -- the counterpart of DCPAM's ProfUtil_RapStart/RapEnd
+ *      regions around the dynamical core (atm/dcpam_main_mod.f90:2167, :2203).
+ *      Only for eager launches (not under graph capture).                   */
+enum {
+  SHT_STAGE_LEG_INV = 0, /* Legendre synthesis GEMMs   */
+  SHT_STAGE_FFT_INV = 1, /* ring synthesis FFT        */
+  SHT_STAGE_FFT_FWD = 2, /* ring analysis FFT         */
+  SHT_STAGE_LEG_FWD = 3, /* Legendre analysis GEMMs   */
+  SHT_NSTAGE = 4
+};
+int sht_plan_set_timing(sht_plan *plan, int enable);
+/* Waits for the recorded launches, returns per-stage total milliseconds and
+ * launch counts (arrays of SHT_NSTAGE; either may be NULL), then resets. */
+int sht_plan_stage_times(sht_plan *plan, double *total_ms, long long *launches);
+
 #ifdef __cplusplus
 }
 #endif

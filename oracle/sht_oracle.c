@@ -30,17 +30,33 @@
 /* Gauss-Legendre nodes: Newton iteration in the colatitude theta, so that both
  * mu = cos(theta) and cos(lat) = sin(theta) keep full relative precision.     */
 
-static void legendre_pn(int n, double x, double *pn, double *pn1) {
-  /* Bonnet recurrence (EBM_main.f90:97-103): returns P_n(x) and P_{n-1}(x). */
-  double p0 = 1.0, p1 = x;
-  if (n == 0) { *pn = 1.0; *pn1 = 0.0; return; }
-  for (int k = 2; k <= n; ++k) {
-    double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
-    p0 = p1;
-    p1 = p2;
+/* P_n and the combination n (P_{n-1} - x P_n) = (1 - x^2) dP_n/dx at
+ * x = cos(theta).  For x > 1/2 Reinsch's difference form of the Bonnet
+ * recurrence (EBM_main.f90:97-103 is the plain form) runs in u = 1 - x =
+ * 2 sin^2(theta/2), computed from theta, so no rounded x enters near the pole:
+ *   D_k = P_k - P_{k-1} = ((k-1) D_{k-1} - (2k-1) u P_{k-1}) / k,  P_k = P_{k-1} + D_k
+ * and P_{n-1} - x P_n = u P_n - D_n.                                          */
+static void legendre_pn_theta(int n, double th, double *pn, double *dq) {
+  double x = cos(th);
+  if (x > 0.5) {
+    double sh = sin(0.5 * th), u = 2.0 * sh * sh;
+    double p = 1.0, d = 0.0; /* P_0, D_0 (unused) */
+    for (int k = 1; k <= n; ++k) {
+      d = ((k - 1.0) * d - (2.0 * k - 1.0) * u * p) / k;
+      p += d;
+    }
+    *pn = p;
+    *dq = n * (u * p - d);
+  } else {
+    double p0 = 1.0, p1 = x;
+    for (int k = 2; k <= n; ++k) {
+      double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    *pn = p1;
+    *dq = n * (p0 - x * p1);
   }
-  *pn = p1;
-  *pn1 = p0;
 }
 
 int orc_gauss(int jm, double *mu, double *w, double *coslat) {
@@ -49,23 +65,24 @@ int orc_gauss(int jm, double *mu, double *w, double *coslat) {
   for (int k = 0; k < h; ++k) {
     /* k-th root counted from the north pole: theta ~ pi (4k+3)/(4jm+2). */
     double th = M_PI * (4.0 * k + 3.0) / (4.0 * jm + 2.0);
-    double pn = 0, pn1 = 0;
+    double pn = 0, dq = 0;
     for (int it = 0; it < 100; ++it) {
-      double x = cos(th), s = sin(th);
-      legendre_pn(jm, x, &pn, &pn1);
-      /* dP/dx = n (P_{n-1} - x P_n) / (1 - x^2);  dP/dtheta = -s dP/dx */
-      double dpdth = -jm * (pn1 - x * pn) / s;
-      double d = pn / dpdth;
+      legendre_pn_theta(jm, th, &pn, &dq);
+      /* dP/dtheta = -sin(theta) dP/dx = -dq / sin(theta) */
+      double d = -pn * sin(th) / dq;
       th -= d;
-      if (fabs(d) < 1e-17 * (1.0 + th)) break;
+      if (fabs(d) < 1e-15 * th) { /* converged: one more step, then stop */
+        legendre_pn_theta(jm, th, &pn, &dq);
+        th -= -pn * sin(th) / dq;
+        break;
+      }
     }
-    double x = cos(th), s = sin(th);
-    legendre_pn(jm, x, &pn, &pn1);
-    double dp = jm * (pn1 - x * pn); /* = (1-x^2) dP/dx */
-    double wk = 2.0 * s * s / (dp * dp);
+    legendre_pn_theta(jm, th, &pn, &dq);
+    double s = sin(th);
+    double wk = 2.0 * s * s / (dq * dq); /* 2 / ((1-x^2) P_n'(x)^2) */
     /* north node index jm-1-k, mirrored south node k */
-    mu[jm - 1 - k] = x;
-    mu[k] = -x;
+    mu[jm - 1 - k] = cos(th);
+    mu[k] = -mu[jm - 1 - k];
     w[jm - 1 - k] = w[k] = wk;
     coslat[jm - 1 - k] = coslat[k] = s;
   }

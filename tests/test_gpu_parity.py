@@ -65,10 +65,9 @@ def test_plan_grid_matches_oracle():
         lon, lat, lw, yw = p.grid()
         mu, w, cl = oracle.gauss(jm)
         assert np.abs(np.sin(lat) - mu).max() < 2e-15
-        # The weight formula 2 sin^2/(n P_{n-1})^2 evaluates P_{n-1} by recurrence;
-        # near the poles both the oracle and the GPU land ~1e-13 relative from the
-        # mpmath weights (the oracle's own pinned accuracy), so they agree to that.
-        assert np.all(np.abs(yw - w) < 5e-13 * w)
+        # both sides are within ~1e-14 relative of mpmath (GPU and host libm
+        # differ in the last bits of sin/cos)
+        assert np.all(np.abs(yw - w) < 2e-14 * w)
         assert np.abs(np.cos(lat) - cl).max() < 1e-14
         assert abs(lw.sum() - 2 * np.pi) < 1e-13
         p.close()
