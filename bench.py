@@ -5,7 +5,7 @@ A step is one forward+inverse pass of the transform over one batch: T85L32
 ln p_s = 97 fields), spectral -> grid -> spectral, with inputs resident in HBM.
 fields/s counts fields that finish the round trip.  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|bf16x3|fp16x3|auto]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|bf16x3|fp16x3|bf16x6|auto]
 """
 from __future__ import annotations
 
@@ -29,14 +29,15 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
 
-PRECISIONS = {"fp64": 0, "bf16x3": 1, "fp16x3": 2}
+PRECISIONS = {"fp64": 0, "bf16x3": 1, "fp16x3": 2, "bf16x6": 3}
 SPIN_CYCLES = 200_000_000  # ~0.1 s: longer than it takes the host to queue K steps
 
 
-def algorithmic(plan_tables_bytes: dict, nf: int):
-    """Per-launch algorithmic bytes and flops of each stage (FP64 I/O)."""
+def algorithmic(plan_tables_bytes: dict, nf: int, four_bytes: int = 16):
+    """Per-launch algorithmic bytes and flops of each stage (FP64 grid and
+    spectra; the Fourier intermediate is complex FP64, or FP32 in mixed plans)."""
     K = (N + 1) * (N + 2) // 2
-    four = 16 * (N + 1) * JM        # Fourier intermediate per field
+    four = four_bytes * (N + 1) * JM  # Fourier intermediate per field
     grid = 8 * IM * JM
     spec = 16 * K
     fft_flops = 2.5 * JM * IM * np.log2(IM)
@@ -50,14 +51,20 @@ def algorithmic(plan_tables_bytes: dict, nf: int):
 
 
 def table_bytes(precision: int):
+    """Bytes of the Legendre tables one launch streams (padded layouts)."""
     P2p = ((JM // 2) + 15) // 16 * 16
+    P2p32 = (P2p + 31) // 32 * 32
+    pieces = {0: 0, 1: 2, 2: 2, 3: 3}[precision]
     inv = fwd = 0
     for m in range(N + 1):
         ke = (N - m + 2) // 2
-        inv += 2 * ((ke + 3) // 4 * 4) * P2p
-        fwd += 2 * P2p * ((ke + 15) // 16 * 16)
-    esz = 8 if precision == 0 else 4  # hi/lo 16-bit pairs
-    return {"inv": inv * esz, "fwd": fwd * esz}
+        if precision == 0:
+            inv += 8 * 2 * ((ke + 3) // 4 * 4) * P2p
+            fwd += 8 * 2 * P2p * ((ke + 15) // 16 * 16)
+        else:
+            inv += 2 * pieces * 2 * P2p * ((ke + 31) // 32 * 32)
+            fwd += 2 * pieces * 2 * ((ke + 15) // 16 * 16) * P2p32
+    return {"inv": inv, "fwd": fwd}
 
 
 def rel(a, b):
@@ -164,12 +171,33 @@ def cpu_baseline(spec_c, seconds: float, threads: int):
     return n * NF / dt, n, dt
 
 
+def traffic_from_profile(form: str, kernel_prefix: str):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3 summary
+    profiles/t85l32_<form>_hbm_traffic.md (FETCH_SIZE/WRITE_SIZE runs scaled by a
+    copy of known size), or None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", f"t85l32_{form}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 7 and cells[0].startswith(kernel_prefix):
+            try:
+                return int(float(cells[6]) * 1e6)
+            except ValueError:
+                return None
+    return None
+
+
+KERNEL_NAMES = {"fft_inv": "fft_inv_kernel", "fft_fwd": "fft_fwd_kernel"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--precision", default="fp64", choices=sorted(PRECISIONS) + ["auto"])
+    ap.add_argument("--precision", default="auto", choices=sorted(PRECISIONS) + ["auto"],
+                    help="auto: run every form; the headline is split-BF16 (six products, <=1e-6)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -179,40 +207,52 @@ def main():
 
     import oracle
 
-    prec_name = "fp64" if args.precision == "auto" else args.precision
-    prec = PRECISIONS[prec_name]
-    res = run_gpu(prec, args.steps, args.warmup)
-    fp64 = res if prec == 0 else run_gpu(0, args.steps, args.warmup)
-
-    # error of this run's outputs against the FP64 oracle on the same inputs
+    forms = list(PRECISIONS) if args.precision == "auto" else sorted({"fp64", args.precision})
+    headline = "bf16x6" if args.precision == "auto" else args.precision
     ref = oracle.Plan(N, IM, JM)
-    g_ref = ref.spec_to_grid(res["spec_in"])
-    err_grid = rel(res["grid"], g_ref)
-    err_spec = rel(res["spec"], ref.grid_to_spec(res["grid"]))
-    err_rt = rel(res["spec"], res["spec_in"])
+    results = {}
+    for form in forms:
+        res = run_gpu(PRECISIONS[form], args.steps, args.warmup)
+        g_ref = ref.spec_to_grid(res["spec_in"])
+        res["err"] = {"grid_rel_l2": rel(res["grid"], g_ref),
+                      "spec_rel_l2": rel(res["spec"], ref.grid_to_spec(res["grid"])),
+                      "roundtrip_rel_l2": rel(res["spec"], res["spec_in"])}
+        res["best_ms"] = min(x for x in (res["eager_ms"], res["graph_ms"]) if x is not None)
+        results[form] = res
 
-    best_ms = min(x for x in (res["eager_ms"], res["graph_ms"]) if x is not None)
-    value = NF / (best_ms * 1e-3)
+    res = results[headline]
+    prec = PRECISIONS[headline]
+    value = NF / (res["best_ms"] * 1e-3)
 
-    # roofline of the dominant kernel
+    # roofline of the dominant kernel of the headline form
     tb = table_bytes(prec)
-    alg = algorithmic(tb, NF)
+    alg = algorithmic(tb, NF, 16 if prec == 0 else 8)
     kms = {k: v for k, v in res["kernel_ms"].items() if v}
     dom = max(kms, key=kms.get)
     bytes_, flops = alg[dom]
     achieved = bytes_ / (kms[dom] * 1e-3) / 1e9
+    kname = KERNEL_NAMES.get(dom, dom + ("_f64_kernel" if prec == 0 else "_mixed_kernel"))
+    traffic = traffic_from_profile(headline, kname)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "kernel_us": round(kms[dom] * 1e3, 2), "algorithmic_bytes": int(bytes_)}
-    kernels = {}
-    for k, ms in kms.items():
-        b, f = alg[k]
-        kernels[k] = {"us": round(ms * 1e3, 2), "GB/s": round(b / (ms * 1e-3) / 1e9, 1),
-                      "TFLOP/s": round(f / (ms * 1e-3) / 1e12, 3)}
-    for k in ("leg_inv", "leg_fwd"):
-        if k in kernels:
-            peak = FP64_PEAK_TFLOPS if prec == 0 else BF16_PEAK_TFLOPS
-            kernels[k]["mfma_frac"] = round(kernels[k]["TFLOP/s"] / peak, 4)
+    per_form = {}
+    for form, r in results.items():
+        t = {"fields_per_s": round(NF / (r["best_ms"] * 1e-3), 1), "ms_per_step": round(r["best_ms"], 5),
+             "eager_ms": round(r["eager_ms"], 5),
+             "graph_ms": None if r["graph_ms"] is None else round(r["graph_ms"], 5),
+             "err_vs_fp64_oracle": r["err"], "kernels_us": {}}
+        a = algorithmic(table_bytes(PRECISIONS[form]), NF, 16 if form == "fp64" else 8)
+        for k, ms in r["kernel_ms"].items():
+            if not ms:
+                continue
+            b, f = a[k]
+            t["kernels_us"][k] = round(ms * 1e3, 2)
+            if k.startswith("leg"):
+                peak = FP64_PEAK_TFLOPS if form == "fp64" else BF16_PEAK_TFLOPS
+                t.setdefault("legendre_mfma_frac", {})[k] = round(f / (ms * 1e-3) / 1e12 / peak, 4)
+        per_form[form] = t
+    fp64_v = per_form["fp64"]["fields_per_s"]
 
     threads = min(16, os.cpu_count() or 1)
     cpu_v, cpu_n, cpu_dt = cpu_baseline(res["spec_in"], args.cpu_seconds, threads)
@@ -225,20 +265,20 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(best_ms, 5),
+        "ms_per_step": round(res["best_ms"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {0: "f64", 1: "bf16x3/f32acc/f64-final", 2: "f16x3/f32acc/f64-final"}[prec],
+        "dtype": {0: "f64", 1: "bf16x2pieces/3products/f32-slice/f64-sum",
+                  2: "f16x2pieces-scaled/3products/f32-slice/f64-sum",
+                  3: "bf16x3pieces/6products/f32-slice/f64-sum"}[prec],
         "data": "synthetic (seeded CN(0,n^-3) spectra, master seed 20261202)",
         "config": {"workload": "T85L32 forward+inverse SHT: 97 fields, 256x128 Gaussian grid, N=85",
-                   "fields_per_step": NF, "precision": prec_name, "timing": "graph" if
-                   res["graph_ms"] is not None and res["graph_ms"] <= res["eager_ms"] else "eager"},
-        "eager_ms_per_step": round(res["eager_ms"], 5),
-        "graph_ms_per_step": None if res["graph_ms"] is None else round(res["graph_ms"], 5),
-        "fp64_fields_per_s": round(NF / (min(x for x in (fp64["eager_ms"], fp64["graph_ms"]) if x) * 1e-3), 1),
-        "err_vs_fp64_oracle": {"grid_rel_l2": err_grid, "spec_rel_l2": err_spec, "roundtrip_rel_l2": err_rt},
-        "kernels": kernels,
+                   "fields_per_step": NF, "precision": headline,
+                   "timing": "graph" if res["graph_ms"] is not None and res["graph_ms"] <= res["eager_ms"] else "eager"},
+        "err_vs_fp64_oracle": res["err"],
+        "speedup_vs_fp64_form": round(value / fp64_v, 3),
+        "forms": per_form,
         "roofline": roofline,
         "cpu_baseline": {"value": round(cpu_v, 1), "unit": "fields/s", "cores": threads, "kind": "port",
                          "sample": f"oracle round trips of the same 97-field T85L32 batch, {cpu_n} reps "

@@ -20,8 +20,8 @@ static int validate_desc(const sht_desc *d) {
   if (!d) return fail(SHT_ERR_INVALID, "sht_plan_create: desc is NULL");
   if (d->nmax < 0 || d->nmax > 255)
     return fail(SHT_ERR_INVALID, "sht_plan_create: nmax must be in [0, 255]");
-  if (d->jm < 2 || (d->jm & 1))
-    return fail(SHT_ERR_INVALID, "sht_plan_create: jm must be even and >= 2");
+  if (d->jm < 2 || (d->jm & 1) || d->jm > 512)
+    return fail(SHT_ERR_INVALID, "sht_plan_create: jm must be even and in [2, 512]");
   if (d->jm <= d->nmax)
     return fail(SHT_ERR_INVALID, "sht_plan_create: jm must exceed nmax (exact quadrature)");
   if (d->im < 2 * d->nmax + 2)
@@ -29,7 +29,7 @@ static int validate_desc(const sht_desc *d) {
   if (!is_pow2(d->im) || d->im < 16 || d->im > 2048)
     return fail(SHT_ERR_UNSUPPORTED, "sht_plan_create: im must be a power of two in [16, 2048]");
   if (d->kmax < 1) return fail(SHT_ERR_INVALID, "sht_plan_create: kmax must be >= 1");
-  if (d->precision < SHT_PREC_FP64 || d->precision > SHT_PREC_SCALED_FP16)
+  if (d->precision < SHT_PREC_FP64 || d->precision > SHT_PREC_SPLIT_BF16X6)
     return fail(SHT_ERR_INVALID, "sht_plan_create: unknown precision");
   if (d->flags & ~SHT_FLAG_GRAD) return fail(SHT_ERR_INVALID, "sht_plan_create: unknown flags");
   const size_t four = (size_t)(d->jm / 2) * (d->nmax + 1) * 4 * (size_t)d->kmax;
@@ -45,7 +45,8 @@ static void release(Plan &p) {
   }
   for (hipEvent_t e : p.event_pool) (void)hipEventDestroy(e);
   void *ptrs[] = {p.mu, p.w, p.coslat, p.twid, p.Pinv, p.Hinv, p.Pfwd, p.poff_inv, p.poff_fwd,
-                  p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.four, p.four2, p.stage_a, p.stage_b};
+                  p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.lexp_inv, p.lexp_fwd, p.four, p.four2,
+                  p.stage_a, p.stage_b};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p.pinned) (void)hipHostFree(p.pinned);
@@ -101,21 +102,25 @@ static int timed(Plan &p, int stage, hipStream_t s, F &&launch) {
 
 static int spec_to_grid(Plan &p, const double *spec, double *grid, int nf, hipStream_t s) {
   int rc = timed(p, SHT_STAGE_LEG_INV, s, [&] {
-    return p.prec == SHT_PREC_FP64 ? launch_leg_inv_f64(p, spec, p.four, p.four2, true, false, nf, s)
-                                   : launch_leg_inv_mixed(p, spec, p.four, nf, s);
+    return p.prec == SHT_PREC_FP64
+               ? launch_leg_inv_f64(p, spec, p.four, p.four2, true, false, nf, s)
+               : launch_leg_inv_mixed(p, spec, reinterpret_cast<float *>(p.four), nf, s);
   });
   if (rc) return rc;
   return timed(p, SHT_STAGE_FFT_INV, s, [&] {
-    return launch_fft_inv(p, p.four, nullptr, grid, nullptr, nullptr, nf, s);
+    return launch_fft_inv(p, p.four, nullptr, p.prec != SHT_PREC_FP64, grid, nullptr, nullptr, nf, s);
   });
 }
 
 static int grid_to_spec(Plan &p, const double *grid, double *spec, int nf, hipStream_t s) {
-  int rc = timed(p, SHT_STAGE_FFT_FWD, s, [&] { return launch_fft_fwd(p, grid, p.four, nf, s); });
+  int rc = timed(p, SHT_STAGE_FFT_FWD, s, [&] {
+    return launch_fft_fwd(p, grid, p.four, p.prec != SHT_PREC_FP64, nf, s);
+  });
   if (rc) return rc;
   return timed(p, SHT_STAGE_LEG_FWD, s, [&] {
-    return p.prec == SHT_PREC_FP64 ? launch_leg_fwd_f64(p, p.four, spec, nf, s)
-                                   : launch_leg_fwd_mixed(p, p.four, spec, nf, s);
+    return p.prec == SHT_PREC_FP64
+               ? launch_leg_fwd_f64(p, p.four, spec, nf, s)
+               : launch_leg_fwd_mixed(p, reinterpret_cast<const float *>(p.four), spec, nf, s);
   });
 }
 
@@ -129,7 +134,7 @@ static int spec_to_grad(Plan &p, const double *spec, double *grid, double *glon,
   });
   if (rc) return rc;
   return timed(p, SHT_STAGE_FFT_INV, s, [&] {
-    return launch_fft_inv(p, p.four, p.four2, grid, glon, glat, nf, s);
+    return launch_fft_inv(p, p.four, p.four2, false, grid, glon, glat, nf, s);
   });
 }
 
@@ -168,7 +173,8 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   rc = build_tables(p);
   if (!rc && p.prec != SHT_PREC_FP64) rc = build_mixed_tables(p);
   if (!rc) {
-    const size_t four = (size_t)p.P2 * (p.N + 1) * 4 * (size_t)p.kmax;
+    // [P2][nfg][N+1][2][2G]: field groups of G = 2048/im are allocated whole
+    const size_t four = (size_t)p.P2 * (p.N + 1) * 4 * (size_t)roundup(p.kmax, fft_group(p.im));
     e = hipMalloc(&p.four, sizeof(double) * four);
     if (e == hipSuccess && (p.flags & SHT_FLAG_GRAD)) e = hipMalloc(&p.four2, sizeof(double) * four);
     if (e != hipSuccess) rc = fail(SHT_ERR_NOMEM, "sht_plan_create: Fourier scratch allocation failed");

@@ -15,8 +15,9 @@
 // layout the Legendre GEMMs want, so this kernel is also the (lat,lon) <->
 // (m,lat) transpose: global traffic is row-contiguous on the grid side and
 // 2G-double runs on the Fourier side; the reordering happens in LDS.
-// The complex FFT is a Stockham autosort (radix 4, one radix-2 pass when log2 L
-// is odd) in a single 32 KB LDS buffer holding G*L = 2048 complex doubles.
+// The complex FFT is a Stockham autosort with radix 8/16 butterflies held in
+// registers (two or three LDS passes per transform) in a single 32 KB LDS
+// buffer holding G*L = 2048 complex doubles.
 #include "sht_internal.h"
 
 namespace sht {
@@ -24,6 +25,16 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLdsComplex = 2048;
+// LDS holds one pad element per 16 complex values: the radix-16 pass writes
+// 16 consecutive outputs per thread (a 256-byte lane stride), which without the
+// pad lands every lane of a ds_write on the same banks.
+constexpr int kLdsPadded = kLdsComplex + kLdsComplex / 16;
+__device__ __forceinline__ int px(int i) { return i + (i >> 4); }
+
+__device__ __forceinline__ void store_pair(double *o, double2 v) { *reinterpret_cast<double2 *>(o) = v; }
+__device__ __forceinline__ void store_pair(float *o, double2 v) {
+  *reinterpret_cast<float2 *>(o) = make_float2((float)v.x, (float)v.y);
+}
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -36,150 +47,231 @@ __device__ __forceinline__ double2 mul_si(double2 a) {
   return SIGN > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
 }
 
-// In-place complex FFT of G = kLdsComplex / L transforms stored back to back in
-// buf; exponent sign SIGN (-1 forward, +1 inverse), unnormalised.  tw holds
-// e^{+2 pi i t / L}.  Ends with a barrier.
-template <int L, int SIGN>
-__device__ __forceinline__ void fft_lds(double2 *buf, const double2 *__restrict__ tw) {
-  constexpr int G = kLdsComplex / L;
-  constexpr int Q4 = G * (L / 4) / kThreads; // radix-4 butterflies per thread
-  const int tid = threadIdx.x;
-  int Ns = 1;
-  for (; Ns * 4 <= L; Ns *= 4) {
-    double2 v[Q4][4];
+// e^{+2 pi i k/16}, k = 0..15
+__constant__ double kW16[16][2] = {
+    {1.0, 0.0}, {0.92387953251128674, 0.38268343236508977}, {0.70710678118654752, 0.70710678118654752},
+    {0.38268343236508977, 0.92387953251128674}, {0.0, 1.0}, {-0.38268343236508977, 0.92387953251128674},
+    {-0.70710678118654752, 0.70710678118654752}, {-0.92387953251128674, 0.38268343236508977}, {-1.0, 0.0},
+    {-0.92387953251128674, -0.38268343236508977}, {-0.70710678118654752, -0.70710678118654752},
+    {-0.38268343236508977, -0.92387953251128674}, {0.0, -1.0}, {0.38268343236508977, -0.92387953251128674},
+    {0.70710678118654752, -0.70710678118654752}, {0.92387953251128674, -0.38268343236508977}};
+
+// In-register DFT of R = 2, 4, 8 or 16 points (exponent sign SIGN), radix-2
+// decimation in time over bit-reversed inputs; every index and constant
+// twiddle is resolved at compile time.
+template <int R, int SIGN>
+__device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
+  constexpr int LOG = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : 4;
+  double2 t[R];
 #pragma unroll
-    for (int q = 0; q < Q4; ++q) {
-      const int idx = tid + q * kThreads;
-      const int g = idx / (L / 4), j = idx % (L / 4), k = j % Ns;
-      const double2 *src = buf + g * L;
+  for (int i = 0; i < R; ++i) {
+    int rev = 0;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) v[q][r] = src[j + r * (L / 4)];
-      const int ts = k * (L / (Ns * 4));
-#pragma unroll
-      for (int r = 1; r < 4; ++r) {
-        double2 w = tw[(r * ts) & (L - 1)];
-        w.y *= SIGN;
-        v[q][r] = cmul(v[q][r], w);
-      }
-      const double2 t0 = cadd(v[q][0], v[q][2]), t1 = csub(v[q][0], v[q][2]);
-      const double2 t2 = cadd(v[q][1], v[q][3]), t3 = mul_si<SIGN>(csub(v[q][1], v[q][3]));
-      v[q][0] = cadd(t0, t2);
-      v[q][2] = csub(t0, t2);
-      v[q][1] = cadd(t1, t3);
-      v[q][3] = csub(t1, t3);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < Q4; ++q) {
-      const int idx = tid + q * kThreads;
-      const int g = idx / (L / 4), j = idx % (L / 4), k = j % Ns;
-      double2 *dst = buf + g * L + (j / Ns) * Ns * 4 + k;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) dst[r * Ns] = v[q][r];
-    }
-    __syncthreads();
+    for (int b = 0; b < LOG; ++b) rev |= ((i >> b) & 1) << (LOG - 1 - b);
+    t[i] = v[rev];
   }
-  if (Ns < L) { // remaining factor 2: Ns == L/2
-    constexpr int Q2 = G * (L / 2) / kThreads;
-    double2 v[Q2][2];
 #pragma unroll
-    for (int q = 0; q < Q2; ++q) {
-      const int idx = tid + q * kThreads;
-      const int g = idx / (L / 2), j = idx % (L / 2);
-      const double2 *src = buf + g * L;
-      double2 a = src[j], b = src[j + L / 2];
-      double2 w = tw[j];
-      w.y *= SIGN;
-      b = cmul(b, w);
-      v[q][0] = cadd(a, b);
-      v[q][1] = csub(a, b);
-    }
-    __syncthreads();
+  for (int len = 2; len <= R; len <<= 1) {
 #pragma unroll
-    for (int q = 0; q < Q2; ++q) {
-      const int idx = tid + q * kThreads;
-      const int g = idx / (L / 2), j = idx % (L / 2);
-      buf[g * L + j] = v[q][0];
-      buf[g * L + j + L / 2] = v[q][1];
+    for (int i = 0; i < R; i += len)
+#pragma unroll
+      for (int k = 0; k < len / 2; ++k) {
+        const int wi = k * (16 / len);
+        const double2 w = make_double2(kW16[wi][0], SIGN * kW16[wi][1]);
+        const double2 b = (k == 0) ? t[i + k + len / 2] : cmul(t[i + k + len / 2], w);
+        const double2 a = t[i + k];
+        t[i + k] = cadd(a, b);
+        t[i + k + len / 2] = csub(a, b);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) v[i] = t[i];
+}
+
+// One Stockham pass of radix R over G transforms of length L in buf, Ns the
+// product of the radices already applied.  Butterfly j of a transform reads
+// x[j + r L/R], applies w^(r k L/(Ns R)) (k = j mod Ns; the powers are built by
+// repeated multiplication from one LDS twiddle), runs a register DFT and
+// writes y[(j/Ns) Ns R + k + r Ns].  Reads finish (barrier) before writes.
+template <int L, int R, int Ns, int SIGN>
+__device__ __forceinline__ void stockham_pass(double2 *buf, const double2 *tw) {
+  constexpr int G = kLdsComplex / L;
+  constexpr int NB = G * (L / R);                       // butterflies in the block
+  constexpr int QB = (NB + kThreads - 1) / kThreads;    // per thread
+  const int tid = threadIdx.x;
+  double2 v[QB][R];
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int idx = tid + q * kThreads;
+    if (NB % kThreads != 0 && idx >= NB) continue;
+    const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[q][r] = buf[px(g * L + j + r * (L / R))];
+    if constexpr (Ns > 1) {
+      double2 w1 = tw[k * (L / (Ns * R))];
+      w1.y *= SIGN;
+      double2 w = w1;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        v[q][r] = cmul(v[q][r], w);
+        if (r + 1 < R) w = cmul(w, w1);
+      }
     }
-    __syncthreads();
+    dft_reg<R, SIGN>(v[q]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int idx = tid + q * kThreads;
+    if (NB % kThreads != 0 && idx >= NB) continue;
+    const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
+    const int d0 = g * L + (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) buf[px(d0 + r * Ns)] = v[q][r];
+  }
+  __syncthreads();
+}
+
+// In-place complex FFT of G = kLdsComplex / L transforms stored back to back in
+// buf; exponent sign SIGN (-1 forward, +1 inverse), unnormalised.  tw (in LDS)
+// holds e^{+2 pi i t / L}.  Radix plan: 16 = 16, 32 = 8*4, 64 = 8*8,
+// 128 = 16*8, 256 = 16*16, 512 = 8*8*8, 1024 = 16*16*4, 2048 = 16*16*8.
+// Ends with a barrier.
+template <int L, int SIGN>
+__device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw) {
+  if constexpr (L == 16) {
+    stockham_pass<L, 16, 1, SIGN>(buf, tw);
+  } else if constexpr (L == 32) {
+    stockham_pass<L, 8, 1, SIGN>(buf, tw);
+    stockham_pass<L, 4, 8, SIGN>(buf, tw);
+  } else if constexpr (L == 64) {
+    stockham_pass<L, 8, 1, SIGN>(buf, tw);
+    stockham_pass<L, 8, 8, SIGN>(buf, tw);
+  } else if constexpr (L == 128) {
+    stockham_pass<L, 16, 1, SIGN>(buf, tw);
+    stockham_pass<L, 8, 16, SIGN>(buf, tw);
+  } else if constexpr (L == 256) {
+    stockham_pass<L, 16, 1, SIGN>(buf, tw);
+    stockham_pass<L, 16, 16, SIGN>(buf, tw);
+  } else if constexpr (L == 512) {
+    stockham_pass<L, 8, 1, SIGN>(buf, tw);
+    stockham_pass<L, 8, 8, SIGN>(buf, tw);
+    stockham_pass<L, 8, 64, SIGN>(buf, tw);
+  } else if constexpr (L == 1024) {
+    stockham_pass<L, 16, 1, SIGN>(buf, tw);
+    stockham_pass<L, 16, 16, SIGN>(buf, tw);
+    stockham_pass<L, 4, 256, SIGN>(buf, tw);
+  } else {
+    static_assert(L == 2048, "unsupported ring length");
+    stockham_pass<L, 16, 1, SIGN>(buf, tw);
+    stockham_pass<L, 16, 16, SIGN>(buf, tw);
+    stockham_pass<L, 8, 256, SIGN>(buf, tw);
   }
 }
 
 // grid -> four (S, A halves).  Blocks: P2 * ceil(nf / G).
-template <int L>
+// All global loads of a thread (its 2 * G*L/256 ring samples and its twiddles)
+// are issued before the first LDS store, so a block pays one HBM latency.
+template <int L, typename TF>
 __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(const double *__restrict__ grid,
-                                                           double *__restrict__ four,
+                                                           TF *__restrict__ four,
                                                            const double2 *__restrict__ tw, int P2,
                                                            int M1, int nf, int jm) {
   constexpr int G = kLdsComplex / L;
-  __shared__ double2 buf[kLdsComplex];
+  constexpr int NQ = kLdsComplex / kThreads; // samples per thread
+  __shared__ double2 buf[kLdsPadded];
+  __shared__ double2 twl[L];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int p = tile % P2, f0 = (tile / P2) * G;
+  const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
+  const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const int tid = threadIdx.x;
-  const size_t ncol = 2 * (size_t)nf;
-  for (int idx = tid; idx < G * L; idx += kThreads) {
-    const int g = idx / L, t = idx % L, f = f0 + g;
-    double2 z = make_double2(0.0, 0.0);
+  double2 z[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int idx = tid + q * kThreads;
+    const int g = idx / L, i = idx % L, f = f0 + g;
+    z[q] = make_double2(0.0, 0.0);
     if (f < nf) {
-      z.x = grid[((size_t)f * jm + jn) * L + t];
-      z.y = grid[((size_t)f * jm + js) * L + t];
+      z[q].x = __builtin_nontemporal_load(grid + ((size_t)f * jm + jn) * L + i);
+      z[q].y = __builtin_nontemporal_load(grid + ((size_t)f * jm + js) * L + i);
     }
-    buf[idx] = z;
   }
+  for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) buf[px(tid + q * kThreads)] = z[q];
   __syncthreads();
-  fft_lds<L, -1>(buf, tw);
+  fft_lds<L, -1>(buf, twl);
   const double sc = 0.5 / L;
   for (int idx = tid; idx < M1 * G; idx += kThreads) {
     const int m = idx / G, g = idx % G, f = f0 + g;
     if (f >= nf) continue;
-    const double2 zm = buf[g * L + m];
-    const double2 zc = buf[g * L + ((L - m) & (L - 1))];
+    const double2 zm = buf[px(g * L + m)];
+    const double2 zc = buf[px(g * L + ((L - m) & (L - 1)))];
     // X_N = (Z_m + conj Z_{L-m}) sc ;  X_S = -i (Z_m - conj Z_{L-m}) sc
     const double2 xn = make_double2((zm.x + zc.x) * sc, (zm.y - zc.y) * sc);
     const double2 xs = make_double2((zm.y + zc.y) * sc, -(zm.x - zc.x) * sc);
     double2 S = cadd(xn, xs), A = csub(xn, xs);
     if (m == 0) S.y = A.y = 0.0;
-    double *o = four + ((size_t)(p * M1 + m) * 2) * ncol + 2 * (size_t)f;
-    *reinterpret_cast<double2 *>(o) = S;
-    *reinterpret_cast<double2 *>(o + ncol) = A;
+    TF *o = four + ((((size_t)p * nfg + fgr) * M1 + m) * 2) * (2 * G) + 2 * g;
+    store_pair(o, S);
+    store_pair(o + 2 * G, A);
   }
 }
 
+
+template <typename TF>
 struct InvOutputs {
-  const double *src[3];
+  const TF *src[3];
   double *dst[3];
   int mode[3]; // 0 value, 1 GradLon, 2 GradLat
 };
 
 // four (even/odd Legendre sums) -> grid.  Blocks: P2 * ceil(nf / G) x noutputs.
-template <int L>
-__global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs io,
+// A thread's Fourier loads (at most 4 (m, field) pairs: M1 * G <= L/2 * G =
+// 1024) and its twiddles are issued before any LDS traffic.
+template <int L, typename TF>
+__global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
                                                            const double2 *__restrict__ tw,
                                                            const double *__restrict__ coslat,
                                                            int P2, int M1, int nf, int jm) {
   constexpr int G = kLdsComplex / L;
-  __shared__ double2 buf[kLdsComplex];
+  constexpr int NQ = kLdsComplex / kThreads;
+  constexpr int NM = (kLdsComplex / 2) / kThreads; // max (m, field) pairs per thread
+  __shared__ double2 buf[kLdsPadded];
+  __shared__ double2 twl[L];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int p = tile % P2, f0 = (tile / P2) * G;
+  const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
+  const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const int tid = threadIdx.x;
   const int sel = blockIdx.y;
-  const double *__restrict__ src = io.src[sel];
+  const TF *__restrict__ src = io.src[sel];
   double *__restrict__ dst = io.dst[sel];
   const int mode = io.mode[sel];
   const double rc = (mode == 0) ? 1.0 : 1.0 / coslat[jn];
-  const size_t ncol = 2 * (size_t)nf;
-  for (int idx = tid; idx < kLdsComplex; idx += kThreads) buf[idx] = make_double2(0.0, 0.0);
-  __syncthreads();
-  for (int idx = tid; idx < M1 * G; idx += kThreads) {
+  double2 E[NM], O[NM];
+#pragma unroll
+  for (int q = 0; q < NM; ++q) {
+    const int idx = tid + q * kThreads;
     const int m = idx / G, g = idx % G, f = f0 + g;
-    if (f >= nf) continue;
-    const double *e = src + ((size_t)(p * M1 + m) * 2) * ncol + 2 * (size_t)f;
-    const double2 E = *reinterpret_cast<const double2 *>(e);
-    const double2 O = *reinterpret_cast<const double2 *>(e + ncol);
-    double2 xn = cadd(E, O), xs = csub(E, O);
+    E[q] = O[q] = make_double2(0.0, 0.0);
+    if (m < M1 && f < nf) {
+      const TF *e = src + ((((size_t)p * nfg + fgr) * M1 + m) * 2) * (2 * G) + 2 * g;
+      E[q] = make_double2((double)e[0], (double)e[1]);
+      O[q] = make_double2((double)e[2 * G], (double)e[2 * G + 1]);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) buf[px(tid + q * kThreads)] = make_double2(0.0, 0.0);
+  for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NM; ++q) {
+    const int idx = tid + q * kThreads;
+    const int m = idx / G, g = idx % G, f = f0 + g;
+    if (m >= M1 || f >= nf) continue;
+    double2 xn = cadd(E[q], O[q]), xs = csub(E[q], O[q]);
     if (mode == 1) { // i m / cos(lat)
       const double s = m * rc;
       xn = make_double2(-xn.y * s, xn.x * s);
@@ -188,48 +280,48 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs io,
       xn = make_double2(xn.x * rc, xn.y * rc);
       xs = make_double2(xs.x * rc, xs.y * rc);
     }
-    double2 *z = buf + g * L;
     if (m == 0) {
-      z[0] = make_double2(xn.x, xs.x);
+      buf[px(g * L)] = make_double2(xn.x, xs.x);
     } else {
-      z[m] = make_double2(xn.x - xs.y, xn.y + xs.x);
-      z[L - m] = make_double2(xn.x + xs.y, -xn.y + xs.x);
+      buf[px(g * L + m)] = make_double2(xn.x - xs.y, xn.y + xs.x);
+      buf[px(g * L + L - m)] = make_double2(xn.x + xs.y, -xn.y + xs.x);
     }
   }
   __syncthreads();
-  fft_lds<L, +1>(buf, tw);
-  for (int idx = tid; idx < G * L; idx += kThreads) {
-    const int g = idx / L, t = idx % L, f = f0 + g;
+  fft_lds<L, +1>(buf, twl);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int idx = tid + q * kThreads;
+    const int g = idx / L, i = idx % L, f = f0 + g;
     if (f >= nf) continue;
-    const double2 z = buf[idx];
-    dst[((size_t)f * jm + jn) * L + t] = z.x;
-    dst[((size_t)f * jm + js) * L + t] = z.y;
+    const double2 z = buf[px(idx)];
+    __builtin_nontemporal_store(z.x, dst + ((size_t)f * jm + jn) * L + i);
+    __builtin_nontemporal_store(z.y, dst + ((size_t)f * jm + js) * L + i);
   }
 }
 
-template <int L>
-int fwd_dispatch(const Plan &p, const double *grid, double *four, int nf, hipStream_t s) {
+template <int L, typename TF>
+int fwd_dispatch(const Plan &p, const double *grid, TF *four, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  fft_fwd_kernel<L><<<nblk, kThreads, 0, s>>>(grid, four, reinterpret_cast<const double2 *>(p.twid),
-                                              p.P2, p.N + 1, nf, p.jm);
+  fft_fwd_kernel<L, TF><<<nblk, kThreads, 0, s>>>(grid, four, reinterpret_cast<const double2 *>(p.twid),
+                                                  p.P2, p.N + 1, nf, p.jm);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
 
-template <int L>
-int inv_dispatch(const Plan &p, const InvOutputs &io, int nout, int nf, hipStream_t s) {
+template <int L, typename TF>
+int inv_dispatch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  fft_inv_kernel<L><<<dim3(nblk, nout), kThreads, 0, s>>>(
+  fft_inv_kernel<L, TF><<<dim3(nblk, nout), kThreads, 0, s>>>(
       io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
 
-}  // namespace
-
-int launch_fft_fwd(const Plan &p, const double *grid, double *four, int nf, hipStream_t s) {
+template <typename TF>
+int fwd_typed(const Plan &p, const double *grid, TF *four, int nf, hipStream_t s) {
   switch (p.im) {
     case 16: return fwd_dispatch<16>(p, grid, four, nf, s);
     case 32: return fwd_dispatch<32>(p, grid, four, nf, s);
@@ -243,9 +335,10 @@ int launch_fft_fwd(const Plan &p, const double *grid, double *four, int nf, hipS
   return fail(SHT_ERR_UNSUPPORTED, "im must be a power of two in [16, 2048]");
 }
 
-int launch_fft_inv(const Plan &p, const double *four, const double *four2, double *grid,
-                   double *gradlon, double *gradlat, int nf, hipStream_t s) {
-  InvOutputs io{};
+template <typename TF>
+int inv_typed(const Plan &p, const TF *four, const TF *four2, double *grid, double *gradlon,
+              double *gradlat, int nf, hipStream_t s) {
+  InvOutputs<TF> io{};
   int n = 0;
   if (grid) { io.src[n] = four; io.dst[n] = grid; io.mode[n] = 0; ++n; }
   if (gradlon) { io.src[n] = four; io.dst[n] = gradlon; io.mode[n] = 1; ++n; }
@@ -262,6 +355,22 @@ int launch_fft_inv(const Plan &p, const double *four, const double *four2, doubl
     case 2048: return inv_dispatch<2048>(p, io, n, nf, s);
   }
   return fail(SHT_ERR_UNSUPPORTED, "im must be a power of two in [16, 2048]");
+}
+
+}  // namespace
+
+int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s) {
+  if (f32) return fwd_typed(p, grid, static_cast<float *>(four), nf, s);
+  return fwd_typed(p, grid, static_cast<double *>(four), nf, s);
+}
+
+int launch_fft_inv(const Plan &p, const void *four, const void *four2, bool f32, double *grid,
+                   double *gradlon, double *gradlat, int nf, hipStream_t s) {
+  if (f32)
+    return inv_typed(p, static_cast<const float *>(four), static_cast<const float *>(four2), grid,
+                     gradlon, gradlat, nf, s);
+  return inv_typed(p, static_cast<const double *>(four), static_cast<const double *>(four2), grid,
+                   gradlon, gradlat, nf, s);
 }
 
 }  // namespace sht

@@ -39,17 +39,16 @@ int fail(int code, const std::string &msg);
 //   KP(m) = roundup(ceil((N-m+1)/2), 4), KR(m) = roundup(ceil((N-m+1)/2), 16),
 //   padding entries are zero.
 //
-//   four  [P2][N+1][par 2][ncol]  the Fourier intermediate: per latitude pair and
-//         wavenumber the equatorially even/odd halves, ncol = 2*nfield doubles
-//         (complex, field innermost).  Forward: (F(mu)+F(-mu), F(mu)-F(-mu));
-//         inverse: the even-n and odd-n Legendre sums (north = even + odd,
-//         south = even - odd).
+//   four  [P2][nfg][N+1][par 2][2G]  the Fourier intermediate: per latitude pair
+//         and wavenumber the equatorially even/odd halves, complex with the
+//         fields of one FFT block (G = 2048/im fields, nfg = ceil(nf/G) groups)
+//         innermost, so each FFT block reads or writes one contiguous, aligned
+//         slice.  Forward: (F(mu)+F(-mu), F(mu)-F(-mu)); inverse: the even-n and
+//         odd-n Legendre sums (north = even + odd, south = even - odd).
 //
-// Mixed-precision tables (precision != FP64) hold the same Pinv/Pfwd entries
-// split into hi/lo 16-bit pieces, one scale exponent per m:
-//   Linv  per m: [par 2][KP16(m)][P2] x {hi, lo} packed as 16-bit pairs
-//   Lfwd  per m: [par 2][P2/32*32][KR(m)] ...
-// (see sht_legendre_mixed.hip).
+// Mixed-precision plans also hold the tables split into 16-bit pieces in the
+// matrix-core operand layout (sht_legendre_mixed.hip) and use an FP32 Fourier
+// intermediate.
 
 struct Plan {
   int N = 0, im = 0, jm = 0, P2 = 0, P2p = 0, K = 0, kmax = 0, prec = 0, flags = 0;
@@ -65,6 +64,7 @@ struct Plan {
   // mixed-precision tables
   void *Linv = nullptr, *Lfwd = nullptr;
   int *loff_inv = nullptr, *loff_fwd = nullptr;
+  int *lexp_inv = nullptr, *lexp_fwd = nullptr; // FP16 power-of-two per (m, parity)
   size_t n_linv = 0, n_lfwd = 0;
   // scratch
   double *four = nullptr, *four2 = nullptr; // [P2][N+1][2][2*kmax]
@@ -82,6 +82,22 @@ struct Plan {
 };
 
 __host__ __device__ inline int roundup(int a, int b) { return (a + b - 1) / b * b; }
+
+// Fourier scratch geometry: G = fields per FFT workgroup, cgs = 2G columns per
+// group (a power of two), column c = 2*field + {re, im}.
+struct FourGeom {
+  int M1, nfg, lg;  // lg = log2(cgs)
+  __host__ __device__ size_t at(int p, int m, int par, int c) const {
+    return ((((size_t)p * nfg + (c >> lg)) * M1 + m) * 2 + par) * ((size_t)1 << lg) + (c & ((1 << lg) - 1));
+  }
+};
+inline int fft_group(int im) { return 2048 / im; }
+inline FourGeom four_geom(int N, int im, int nf) {
+  const int G = fft_group(im);
+  int lg = 0;
+  while ((1 << lg) < 2 * G) ++lg;
+  return FourGeom{N + 1, (nf + G - 1) / G, lg};
+}
 __host__ __device__ inline int ceil_half(int a) { return (a + 1) / 2; }
 __host__ __device__ inline int64_t coef_off(int N, int m) {
   return (int64_t)m * (N + 1) - (int64_t)m * (m - 1) / 2;
@@ -91,9 +107,10 @@ __host__ __device__ inline int64_t coef_off(int N, int m) {
 // Table construction (sht_tables.hip)
 int build_tables(Plan &p);
 
-// FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos)
-int launch_fft_fwd(const Plan &p, const double *grid, double *four, int nf, hipStream_t s);
-int launch_fft_inv(const Plan &p, const double *four, const double *four2, double *grid,
+// FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos).
+// f32 = the Fourier intermediate is FP32 (mixed-precision plans); the FFT itself is FP64.
+int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s);
+int launch_fft_inv(const Plan &p, const void *four, const void *four2, bool f32, double *grid,
                    double *gradlon, double *gradlat, int nf, hipStream_t s);
 
 // Legendre stage, FP64 MFMA (sht_legendre.hip)
@@ -103,8 +120,8 @@ int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, 
 
 // Legendre stage, mixed precision (sht_legendre_mixed.hip)
 int build_mixed_tables(Plan &p);
-int launch_leg_inv_mixed(const Plan &p, const double *spec, double *four, int nf, hipStream_t s);
-int launch_leg_fwd_mixed(const Plan &p, const double *four, double *spec, int nf, hipStream_t s);
+int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s);
+int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s);
 
 // Layout helpers (sht_pack.hip)
 int launch_spml_to_canon(const Plan &p, const double *wa, double *spec, int nf, hipStream_t s);

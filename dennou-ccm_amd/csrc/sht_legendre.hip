@@ -9,7 +9,7 @@
 // where c runs over 2 * nfield real columns (complex spectra, fields
 // innermost), so levels x fields are the GEMMs' batch dimension.
 //
-// Each workgroup is one wavefront computing one (m, 16-column tile) with
+// Each workgroup computes one (m, 16-column tile) with
 // v_mfma_f64_16x16x4f64: A = table fragment (16 rows x 4; lane l holds row
 // l%16, k l/16), B = data fragment (4 x 16 columns; lane l holds k l/16,
 // column l%16), D = 16 x 16 accumulators, 4 doubles per lane: item v of lane l
@@ -28,23 +28,32 @@ __device__ __forceinline__ d4 mfma_f64(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+constexpr int kWaves = 4;   // wavefronts per workgroup, splitting the GEMM's K dimension
+constexpr int kSteps = 4;   // K-steps (of 4) per wave whose fragments are loaded at once
+
 // Inverse: spec -> four (tab 0, value table) and/or four2 (tab 1, derivative
 // table, written with the parity slots swapped because (1-mu^2) dPt/dmu has the
-// opposite equatorial parity to Pt).  RT row tiles (16 pairs each) per pass.
+// opposite equatorial parity to Pt).
+// Workgroup = one (m, 16-column tile); wave w takes the K-steps w, w+4, ...,
+// issues all their table and spectral fragments at once, runs its MFMAs over
+// RT row tiles (16 latitude pairs each), and the four partial sums are added
+// in FP64 through LDS.
 template <int RT>
-__global__ __launch_bounds__(64) void leg_inv_f64_kernel(
+__global__ __launch_bounds__(256) void leg_inv_f64_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
     const double *__restrict__ Pinv, const double *__restrict__ Hinv,
-    const int *__restrict__ poff, int N, int P2, int P2p, int ncol, int nct, int tab0, int ntab) {
+    const int *__restrict__ poff, int N, int P2, int P2p, int ncol, int nct, int tab0, int ntab,
+    FourGeom geo) {
+  extern __shared__ double red[]; // [kWaves][RT][4][64]
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int ct = b % nct;
   const int rest = b / nct;
   const int tab = tab0 + rest % ntab;
   const int m = rest / ntab;
   const int M1 = N + 1;
-  const int lane = threadIdx.x, r16 = lane & 15, kq = lane >> 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
-  const int KP = (ke + 3) & ~3;
+  const int KP = (ke + 3) & ~3, nsteps = KP >> 2;
   const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m];
   double *__restrict__ out = tab ? four2 : four;
   const int se = tab ? 1 : 0, so = 1 - se;
@@ -53,92 +62,128 @@ __global__ __launch_bounds__(64) void leg_inv_f64_kernel(
   const bool cok = col < ncol;
 
   for (int rt0 = 0; rt0 < P2p / 16; rt0 += RT) {
-    d4 ae[RT], ao[RT];
+    // both parities' fragments are issued in the same round of loads
+    d4 acc[2][RT];
 #pragma unroll
-    for (int r = 0; r < RT; ++r) {
-      ae[r] = d4{0.0, 0.0, 0.0, 0.0};
-      ao[r] = d4{0.0, 0.0, 0.0, 0.0};
-    }
-    const double *te = T + (size_t)kq * P2p + rt0 * 16 + r16;
-    const double *to = te + (size_t)KP * P2p;
-#pragma unroll 2
-    for (int k4 = 0; k4 < KP; k4 += 4) {
-      const int k = k4 + kq;
-      const double be = (cok && k < ke) ? sm[(size_t)(2 * k) * ncol + col] : 0.0;
-      const double bo = (cok && k < ko) ? sm[(size_t)(2 * k + 1) * ncol + col] : 0.0;
-      double a_e[RT], a_o[RT];
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        a_e[r] = te[(size_t)k4 * P2p + r * 16];
-        a_o[r] = to[(size_t)k4 * P2p + r * 16];
+      for (int r = 0; r < RT; ++r) acc[q][r] = d4{0.0, 0.0, 0.0, 0.0};
+    const double *__restrict__ Tp = T + rt0 * 16 + r16;
+    const double *__restrict__ sp = sm + col;
+    for (int j0 = 0; wave + kWaves * j0 < nsteps; j0 += kSteps) {
+      double a[2][kSteps][RT], bv[2][kSteps];
+#pragma unroll
+      for (int j = 0; j < kSteps; ++j) {
+        const int st = wave + kWaves * (j0 + j), k = st * 4 + kq;
+        const bool ok = st < nsteps;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          bv[q][j] = (ok && cok && k < (q ? ko : ke)) ? sp[(size_t)(2 * k + q) * ncol] : 0.0;
+#pragma unroll
+          for (int r = 0; r < RT; ++r)
+            a[q][j][r] = ok ? Tp[(size_t)(q * KP + k) * P2p + r * 16] : 0.0;
+        }
       }
 #pragma unroll
-      for (int r = 0; r < RT; ++r) {
-        ae[r] = mfma_f64(a_e[r], be, ae[r]);
-        ao[r] = mfma_f64(a_o[r], bo, ao[r]);
-      }
+      for (int j = 0; j < kSteps; ++j)
+        if (wave + kWaves * (j0 + j) < nsteps)
+#pragma unroll
+          for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < RT; ++r) acc[q][r] = mfma_f64(a[q][j][r], bv[q][j], acc[q][r]);
     }
-    if (cok) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
 #pragma unroll
       for (int r = 0; r < RT; ++r)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int p = (rt0 + r) * 16 + i * 4 + kq;
-          if (p < P2) {
-            double *dst = out + ((size_t)(p * M1 + m) * 2) * ncol + col;
-            dst[se * ncol] = ae[r][i];
-            dst[so * ncol] = ao[r][i];
-          }
-        }
+        for (int i = 0; i < 4; ++i) red[((wave * RT + r) * 4 + i) * 64 + lane] = acc[q][r][i];
+      __syncthreads();
+      const int slot = q ? so : se;
+      for (int e = threadIdx.x; e < RT * 256; e += 256) {
+        const double v = red[e] + red[e + RT * 256] + red[e + 2 * RT * 256] + red[e + 3 * RT * 256];
+        const int ln = e & 63, ri = e >> 6, i = ri & 3, r = ri >> 2;
+        const int p = (rt0 + r) * 16 + i * 4 + (ln >> 4), c = ct * 16 + (ln & 15);
+        if (p < P2 && c < ncol) out[geo.at(p, m, slot, c)] = v;
+      }
+      __syncthreads();
     }
   }
 }
 
-// Forward: four (S, A halves) -> spec.  MAXRT = row tiles of 16 degrees per
-// parity at m = 0.
+// Forward: four (S, A halves) -> spec.  Workgroup = one (m, 16-column tile);
+// wave w takes the latitude-pair steps w, w+4, ... (4 pairs each) of the
+// quadrature, the partial sums over its pairs meet in LDS and are added in
+// FP64.  MAXRT = row tiles of 16 degrees per parity at m = 0.
 template <int MAXRT>
-__global__ __launch_bounds__(64) void leg_fwd_f64_kernel(const double *__restrict__ four,
-                                                         double *__restrict__ spec,
-                                                         const double *__restrict__ Pfwd,
-                                                         const int *__restrict__ poff, int N,
-                                                         int P2, int P2p, int ncol, int nct) {
+__global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restrict__ four,
+                                                          double *__restrict__ spec,
+                                                          const double *__restrict__ Pfwd,
+                                                          const int *__restrict__ poff, int N,
+                                                          int P2, int P2p, int ncol, int nct,
+                                                          FourGeom geo) {
+  extern __shared__ double red[]; // [kWaves][MAXRT][4][64]
   const int b = xcd_tile(blockIdx.x, gridDim.x);
   const int ct = b % nct, m = b / nct;
   const int M1 = N + 1;
-  const int lane = threadIdx.x, r16 = lane & 15, kq = lane >> 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
+  const int nsteps = P2p >> 2;
   const int col = ct * 16 + r16;
   const bool cok = col < ncol;
   double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol;
-#pragma unroll 1
-  for (int par = 0; par < 2; ++par) {
-    const int kc = par ? (nn >> 1) : ke;
-    d4 acc[MAXRT];
+  const int kcnt[2] = {ke, nn >> 1};
+  d4 acc[2][MAXRT];
 #pragma unroll
-    for (int r = 0; r < MAXRT; ++r) acc[r] = d4{0.0, 0.0, 0.0, 0.0};
-    const double *__restrict__ W = Pfwd + poff[m] + (size_t)par * P2p * KR + (size_t)kq * KR + r16;
-    const double *__restrict__ src = four + (size_t)par * ncol + col;
-#pragma unroll 2
-    for (int p4 = 0; p4 < P2p; p4 += 4) {
-      const int p = p4 + kq;
-      const double bv = (cok && p < P2) ? src[(size_t)(p * M1 + m) * 2 * ncol] : 0.0;
-      double a[MAXRT];
+  for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int r = 0; r < MAXRT; ++r) a[r] = (r < nrt) ? W[(size_t)p4 * KR + r * 16] : 0.0;
+    for (int r = 0; r < MAXRT; ++r) acc[q][r] = d4{0.0, 0.0, 0.0, 0.0};
+  const double *__restrict__ W = Pfwd + poff[m] + r16;
+  const int csafe = cok ? col : 0;
+  // Loads go to clamped, always-valid addresses and are masked afterwards, so
+  // the compiler issues all of them before a single wait (guarded loads become
+  // branches with a wait each).
+  for (int j0 = 0; wave + kWaves * j0 < nsteps; j0 += kSteps) {
+    double a[2][kSteps][MAXRT], bv[2][kSteps];
 #pragma unroll
-      for (int r = 0; r < MAXRT; ++r)
-        if (r < nrt) acc[r] = mfma_f64(a[r], bv, acc[r]);
+    for (int j = 0; j < kSteps; ++j) {
+      const int st = wave + kWaves * (j0 + j), p = st * 4 + kq;
+      const bool ok = st < nsteps;
+      const bool vb = ok && cok && p < P2;
+      const int pb = vb ? p : 0, pa = ok ? p : 0;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const double x = four[geo.at(pb, m, q, csafe)];
+        bv[q][j] = vb ? x : 0.0;
+#pragma unroll
+        for (int r = 0; r < MAXRT; ++r)
+          a[q][j][r] = W[((size_t)q * P2p + pa) * KR + (r < nrt ? r : 0) * 16];
+      }
     }
-    if (cok) {
 #pragma unroll
-      for (int r = 0; r < MAXRT; ++r)
+    for (int j = 0; j < kSteps; ++j)
+      if (wave + kWaves * (j0 + j) < nsteps)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int k = r * 16 + i * 4 + kq;
-          if (r < nrt && k < kc) sm[(size_t)(2 * k + par) * ncol + col] = acc[r][i];
-        }
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int r = 0; r < MAXRT; ++r)
+            if (r < nrt) acc[q][r] = mfma_f64(a[q][j][r], bv[q][j], acc[q][r]);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int r = 0; r < MAXRT; ++r)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[((wave * MAXRT + r) * 4 + i) * 64 + lane] = acc[q][r][i];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nrt * 256; e += 256) {
+      const double v = red[e] + red[e + MAXRT * 256] + red[e + 2 * MAXRT * 256] + red[e + 3 * MAXRT * 256];
+      const int ln = e & 63, ri = e >> 6, i = ri & 3, r = ri >> 2;
+      const int k = r * 16 + i * 4 + (ln >> 4), c = ct * 16 + (ln & 15);
+      if (k < kcnt[q] && c < ncol) sm[(size_t)(2 * k + q) * ncol + c] = v;
     }
+    __syncthreads();
   }
 }
 
@@ -147,8 +192,10 @@ int inv_launch(const Plan &p, const double *spec, double *four, double *four2, i
                int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int nblk = (p.N + 1) * ntab * nct;
-  leg_inv_f64_kernel<RT><<<nblk, 64, 0, s>>>(spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.N,
-                                             p.P2, p.P2p, ncol, nct, tab0, ntab);
+  const size_t lds = sizeof(double) * kWaves * RT * 256;
+  leg_inv_f64_kernel<RT><<<nblk, 256, lds, s>>>(spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.N,
+                                                p.P2, p.P2p, ncol, nct, tab0, ntab,
+                                                four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -157,8 +204,9 @@ template <int MAXRT>
 int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int nblk = (p.N + 1) * nct;
-  leg_fwd_f64_kernel<MAXRT><<<nblk, 64, 0, s>>>(four, spec, p.Pfwd, p.poff_fwd, p.N, p.P2, p.P2p,
-                                                ncol, nct);
+  const size_t lds = sizeof(double) * kWaves * MAXRT * 256;
+  leg_fwd_f64_kernel<MAXRT><<<nblk, 256, lds, s>>>(four, spec, p.Pfwd, p.poff_fwd, p.N, p.P2,
+                                                   p.P2p, ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -171,9 +219,18 @@ int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *
   const int tab0 = value_table ? 0 : 1;
   const int ntab = (value_table && deriv_table) ? 2 : 1;
   const int rtiles = p.P2p / 16;
-  if (rtiles % 4 == 0) return inv_launch<4>(p, spec, four, four2, tab0, ntab, nf, s);
-  if (rtiles % 2 == 0) return inv_launch<2>(p, spec, four, four2, tab0, ntab, nf, s);
-  return inv_launch<1>(p, spec, four, four2, tab0, ntab, nf, s);
+  int rt = 8;
+  while (rtiles % rt) --rt;
+  switch (rt) {
+    case 8: return inv_launch<8>(p, spec, four, four2, tab0, ntab, nf, s);
+    case 7: return inv_launch<7>(p, spec, four, four2, tab0, ntab, nf, s);
+    case 6: return inv_launch<6>(p, spec, four, four2, tab0, ntab, nf, s);
+    case 5: return inv_launch<5>(p, spec, four, four2, tab0, ntab, nf, s);
+    case 4: return inv_launch<4>(p, spec, four, four2, tab0, ntab, nf, s);
+    case 3: return inv_launch<3>(p, spec, four, four2, tab0, ntab, nf, s);
+    case 2: return inv_launch<2>(p, spec, four, four2, tab0, ntab, nf, s);
+    default: return inv_launch<1>(p, spec, four, four2, tab0, ntab, nf, s);
+  }
 }
 
 int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {

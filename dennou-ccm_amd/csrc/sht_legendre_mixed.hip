@@ -1,18 +1,485 @@
-// sht_legendre_mixed.hip -- mixed-precision Legendre step (split-BF16 and
-// per-m scaled FP16 on MFMA).  Not built yet: plan creation reports
-// SHT_ERR_UNSUPPORTED for these precisions.
+// sht_legendre_mixed.hip -- the Gauss-Legendre step on the 16-bit matrix
+// cores (v_mfma_f32_16x16x32_bf16 / _f16) with split operands.
+//
+// Each FP64 operand x is carried as a sum of 16-bit pieces:
+//   SPLIT_BF16    x ~ hi + lo               (bf16, |x - hi - lo| <= 2^-18 |x|)
+//   SPLIT_BF16X6  x ~ hi + mid + lo         (bf16, <= 2^-27 |x|)
+//   SCALED_FP16   x 2^e ~ hi + lo 2^-11     (fp16, <= 2^-22 |x 2^e|)
+// and a product A.B is formed from the piece products whose magnitude is not
+// below the split residual: hi.hi + hi.lo + lo.hi (three products) for the
+// two-piece forms, hi.hi + hi.mid + mid.hi + hi.lo + lo.hi + mid.mid (six) for
+// BF16X6.  Each v_mfma covers a 32-wide slice of the GEMM's K dimension and
+// sums in FP32; the slices are combined (and FP16 scalings undone) in FP64.
+//
+// FP16's exponent range is why that form is scaled: the table block of each
+// (m, parity) carries one power of two (plan time), and the data of each
+// (column, K slice) another, chosen in the kernel from the slice's largest
+// magnitude when the workgroup splits its data tile into LDS.  The lo piece is
+// stored 2^11 larger so it stays a normal FP16 number; its products are kept
+// in their own accumulator.
+//
+// Table layouts (16-bit, K contiguous so one lane loads its 8 K values with a
+// single 16-byte load, matching the 16x16x32 operand layout: lane l holds row
+// l%16, K = 8*(l/16) .. +7):
+//   Linv per m: [par][piece][P2p pair rows][KP32 degrees]
+//   Lfwd per m: [par][piece][KR degree rows][P2p32 pairs]   (w/2 folded in)
+// The Fourier intermediate of mixed-precision plans is FP32.
+#include <cstdint>
+
 #include "sht_internal.h"
 
 namespace sht {
+namespace {
 
-int build_mixed_tables(Plan &) {
-  return fail(SHT_ERR_UNSUPPORTED, "mixed-precision Legendre forms are not built yet");
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int MODE>
+struct Form;
+template <>
+struct Form<SHT_PREC_SPLIT_BF16> {
+  static constexpr int NP = 2;
+  static constexpr bool kScaled = false;
+};
+template <>
+struct Form<SHT_PREC_SPLIT_BF16X6> {
+  static constexpr int NP = 3;
+  static constexpr bool kScaled = false;
+};
+template <>
+struct Form<SHT_PREC_SCALED_FP16> {
+  static constexpr int NP = 2;
+  static constexpr bool kScaled = true;
+};
+
+__device__ __forceinline__ uint16_t bf16_bits(double x) {
+  __bf16 h = (__bf16)(float)x;
+  return __builtin_bit_cast(uint16_t, h);
 }
-int launch_leg_inv_mixed(const Plan &, const double *, double *, int, hipStream_t) {
-  return fail(SHT_ERR_UNSUPPORTED, "mixed-precision Legendre forms are not built yet");
+__device__ __forceinline__ double bf16_val(uint16_t b) {
+  return (double)(float)__builtin_bit_cast(__bf16, b);
 }
-int launch_leg_fwd_mixed(const Plan &, const double *, double *, int, hipStream_t) {
-  return fail(SHT_ERR_UNSUPPORTED, "mixed-precision Legendre forms are not built yet");
+__device__ __forceinline__ uint16_t f16_bits(double x) {
+  _Float16 h = (_Float16)(float)x;
+  return __builtin_bit_cast(uint16_t, h);
+}
+__device__ __forceinline__ double f16_val(uint16_t b) {
+  return (double)(float)__builtin_bit_cast(_Float16, b);
+}
+
+// Splits x (already scaled for FP16) into NP 16-bit pieces.
+template <int MODE>
+__device__ __forceinline__ void split(double x, uint16_t (&pc)[3]) {
+  if constexpr (MODE == SHT_PREC_SCALED_FP16) {
+    pc[0] = f16_bits(x);
+    pc[1] = f16_bits((x - f16_val(pc[0])) * 2048.0);
+    pc[2] = 0;
+  } else {
+    pc[0] = bf16_bits(x);
+    double r = x - bf16_val(pc[0]);
+    pc[1] = bf16_bits(r);
+    pc[2] = bf16_bits(r - bf16_val(pc[1]));
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ f4 mfma16(const uint4 &a, const uint4 &b, f4 c) {
+  if constexpr (MODE == SHT_PREC_SCALED_FP16)
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                  __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+}
+
+// One K slice: adds the slice's contribution, unscaled, to acc in FP64.
+template <int MODE>
+__device__ __forceinline__ void slice_mac(const uint4 (&pa)[3], const uint4 (&pb)[3], double scale,
+                                          d4 &acc) {
+  const f4 z = {0.f, 0.f, 0.f, 0.f};
+  if constexpr (MODE == SHT_PREC_SCALED_FP16) {
+    f4 c1 = mfma16<MODE>(pa[0], pb[1], z);
+    c1 = mfma16<MODE>(pa[1], pb[0], c1);
+    f4 c0 = mfma16<MODE>(pa[0], pb[0], z);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += ((double)c0[i] + (double)c1[i] * (1.0 / 2048.0)) * scale;
+  } else if constexpr (MODE == SHT_PREC_SPLIT_BF16X6) {
+    f4 c = mfma16<MODE>(pa[1], pb[1], z);
+    c = mfma16<MODE>(pa[0], pb[2], c);
+    c = mfma16<MODE>(pa[2], pb[0], c);
+    c = mfma16<MODE>(pa[0], pb[1], c);
+    c = mfma16<MODE>(pa[1], pb[0], c);
+    c = mfma16<MODE>(pa[0], pb[0], c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += (double)c[i];
+  } else {
+    f4 c = mfma16<MODE>(pa[0], pb[1], z);
+    c = mfma16<MODE>(pa[1], pb[0], c);
+    c = mfma16<MODE>(pa[0], pb[0], c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] += (double)c[i];
+  }
+}
+
+// Data tile of one workgroup: for each parity and K slice, the 16 columns x 32
+// K values split into pieces, laid out [group][piece][col][32] (16-bit) so a
+// lane's 8 K values are one 16-byte LDS read; ebits[group][col] holds the FP16
+// scaling exponent.  group = par * maxsl + slice.
+// Thread t converts column t % 16 of group t / 16: it reads its 32 values
+// (load(k) returns the FP64 value or 0 outside the tile), takes their maximum
+// for the exponent, splits and stores them.
+template <int MODE, class Load>
+__device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Load load) {
+  constexpr int NP = Form<MODE>::NP;
+  const int t = threadIdx.x, c = t & 15, g = t >> 4;
+  if (g >= ngroups) return;
+  double v[32];
+  double mx = 0.0;
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    v[k] = load(g, k, c);
+    mx = fmax(mx, fabs(v[k]));
+  }
+  const int e = (Form<MODE>::kScaled && mx > 0.0) ? 13 - ilogb(mx) : 0;
+  ebits[g * 16 + c] = e;
+  uint16_t *dst = bt + ((size_t)g * NP * 16 + c) * 32;
+#pragma unroll
+  for (int k = 0; k < 32; k += 2) {
+    uint16_t p0[3], p1[3];
+    split<MODE>(Form<MODE>::kScaled ? ldexp(v[k], e) : v[k], p0);
+    split<MODE>(Form<MODE>::kScaled ? ldexp(v[k + 1], e) : v[k + 1], p1);
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+      *reinterpret_cast<uint32_t *>(dst + (size_t)q * 16 * 32 + k) = p0[q] | ((uint32_t)p1[q] << 16);
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void load_pb(const uint16_t *bt, int g, int kq, int r16, uint4 (&pb)[3]) {
+  constexpr int NP = Form<MODE>::NP;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    pb[q] = (q < NP) ? *reinterpret_cast<const uint4 *>(bt + (((size_t)g * NP + q) * 16 + r16) * 32 + kq * 8)
+                     : make_uint4(0, 0, 0, 0);
+}
+
+// NSL (template) = the kernel's K slices of 32 at m = 0: KP32/32 <= 4 for the
+// inverse (N <= 255), P2p32/32 <= 8 for the forward (jm <= 512).  Registers and
+// LDS are sized by it, so small truncations keep a high occupancy.
+
+// Inverse: spec (FP64) -> four (FP32 even/odd Legendre sums).
+// Workgroup = one (m, 16-column tile).  Each wave first issues the table
+// fragments of its first 16-pair row tile, then all 256 threads split the
+// spectral tile into LDS; wave w then takes row tiles w, w+4, ...
+template <int MODE, int NSL>
+__global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
+    const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
+    const int *__restrict__ loff, const int *__restrict__ lexp, int N, int P2, int P2p, int ncol,
+    int nct, FourGeom geo) {
+  constexpr int NP = Form<MODE>::NP;
+  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * 32];
+  __shared__ int ebits[2 * NSL * 16];
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int ct = b % nct, m = b / nct;
+  const int M1 = N + 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
+  const int col0 = ct * 16;
+  const double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + col0;
+  const uint16_t *__restrict__ Lm = L + loff[m];
+  const int rtiles = P2p >> 4;
+
+  auto a_frag = [&](int rt, int par, int s, int q) {
+    return *reinterpret_cast<const uint4 *>(
+        Lm + ((size_t)(par * NP + q) * P2p + rt * 16 + r16) * KP32 + s * 32 + kq * 8);
+  };
+  uint4 pa[2][NSL][3];
+  if (wave < rtiles) {
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+      for (int s = 0; s < NSL; ++s)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          pa[par][s][q] = (q < NP) ? a_frag(wave, par, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
+  }
+  stage_data<MODE>(bt, ebits, 2 * NSL, [&](int g, int k, int c) {
+    const int par = g / NSL, s = g % NSL, kk = s * 32 + k;
+    const bool v = s < nsl && kk < (par ? ko : ke) && col0 + c < ncol;
+    const double x = sm[(size_t)(v ? 2 * kk + par : 0) * ncol + (v ? c : 0)];
+    return v ? x : 0.0;
+  });
+  __syncthreads();
+  const int col = col0 + r16;
+  for (int rt = wave; rt < rtiles; rt += 4) {
+    if (rt != wave) {
+#pragma unroll
+      for (int par = 0; par < 2; ++par)
+#pragma unroll
+        for (int s = 0; s < NSL; ++s)
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            pa[par][s][q] = (q < NP) ? a_frag(rt, par, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < NSL; ++s) {
+        if (s >= nsl) break;
+        const int g = par * NSL + s;
+        uint4 pb[3];
+        load_pb<MODE>(bt, g, kq, r16, pb);
+        slice_mac<MODE>(pa[par][s], pb, Form<MODE>::kScaled ? ea * ldexp(1.0, -ebits[g * 16 + r16]) : 1.0, acc);
+      }
+      if (col < ncol) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int p = rt * 16 + kq * 4 + i;
+          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[i];
+        }
+      }
+    }
+  }
+}
+
+// Forward: four (FP32 S/A halves) -> spec (FP64).  Workgroup = one (m,
+// 16-column tile); the S/A tile (all latitude pairs, both parities) is split
+// once into LDS; wave w takes the (parity, 16-degree row tile) units w, w+4, ...
+template <int MODE, int NSL>
+__global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
+    const float *__restrict__ four, double *__restrict__ spec, const uint16_t *__restrict__ L,
+    const int *__restrict__ loff, const int *__restrict__ lexp, int N, int P2, int P2p32, int ncol,
+    int nct, FourGeom geo) {
+  constexpr int NP = Form<MODE>::NP;
+  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * 32];
+  __shared__ int ebits[2 * NSL * 16];
+  const int b = xcd_tile(blockIdx.x, gridDim.x);
+  const int ct = b % nct, m = b / nct;
+  const int M1 = N + 1;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1;
+  const int KR = (ke + 15) & ~15, nrt = KR >> 4;
+  const int nsl = P2p32 >> 5;
+  const int col0 = ct * 16;
+  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + col0;
+  const uint16_t *__restrict__ Lm = L + loff[m];
+  const int nunits = 2 * nrt;
+
+  auto a_frag = [&](int u, int s, int q) {
+    const int par = u / nrt, rt = u % nrt;
+    return *reinterpret_cast<const uint4 *>(
+        Lm + ((size_t)(par * NP + q) * KR + rt * 16 + r16) * P2p32 + s * 32 + kq * 8);
+  };
+  uint4 pa[NSL][3];
+  if (wave < nunits) {
+#pragma unroll
+    for (int s = 0; s < NSL; ++s)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        pa[s][q] = (q < NP) ? a_frag(wave, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
+  }
+  stage_data<MODE>(bt, ebits, 2 * nsl, [&](int g, int k, int c) {
+    const int par = g / nsl, s = g % nsl, p = s * 32 + k;
+    const bool v = p < P2 && col0 + c < ncol;
+    const float x = four[geo.at(v ? p : 0, m, par, col0 + (v ? c : 0))];
+    return v ? (double)x : 0.0;
+  });
+  __syncthreads();
+  for (int u = wave; u < nunits; u += 4) {
+    const int par = u / nrt, rt = u % nrt;
+    const int kc = par ? (nn >> 1) : ke;
+    if (u != wave) {
+#pragma unroll
+      for (int s = 0; s < NSL; ++s)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          pa[s][q] = (q < NP) ? a_frag(u, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
+    }
+    const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NSL; ++s) {
+      if (s >= nsl) break;
+      const int g = par * nsl + s;
+      uint4 pb[3];
+      load_pb<MODE>(bt, g, kq, r16, pb);
+      slice_mac<MODE>(pa[s], pb, Form<MODE>::kScaled ? ea * ldexp(1.0, -ebits[g * 16 + r16]) : 1.0, acc);
+    }
+    if (col0 + r16 < ncol) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = rt * 16 + kq * 4 + i;
+        if (k < kc) sm[(size_t)(2 * k + par) * ncol + r16] = acc[i];
+      }
+    }
+  }
+}
+
+// Plan time: split and transpose one (m, parity) block of an FP64 table into
+// the 16-bit piece layout.  For SCALED_FP16 the block's largest magnitude sets
+// the power of two e (stored value = x 2^e), written to lexp[2m + par].
+// inv = 1: src [KP][P2p] (row k, pair fastest)  -> dst [piece][P2p][KP32]
+// inv = 0: src [P2p][KR] (row pair, k fastest)  -> dst [piece][KR][P2p32]
+template <int MODE>
+__global__ __launch_bounds__(256) void split_table_kernel(const double *__restrict__ src_tab,
+                                                          const int *__restrict__ poff,
+                                                          uint16_t *__restrict__ dst_tab,
+                                                          const int *__restrict__ loff,
+                                                          int *__restrict__ lexp, int N, int P2p,
+                                                          int P2p32, int inv) {
+  constexpr int NP = Form<MODE>::NP;
+  const int m = blockIdx.x >> 1, par = blockIdx.x & 1;
+  const int ke = ceil_half(N - m + 1);
+  const int KP = roundup(ke, 4), KP32 = roundup(ke, 32), KR = roundup(ke, 16);
+  const int rows = inv ? KP : P2p, cols = inv ? P2p : KR;  // source block shape
+  const double *__restrict__ src = src_tab + poff[m] + (size_t)par * rows * cols;
+  const int drows = inv ? P2p : KR, dcols = inv ? KP32 : P2p32;
+  uint16_t *__restrict__ dst = dst_tab + loff[m] + (size_t)par * NP * drows * dcols;
+  __shared__ double red[256];
+  __shared__ int e_sh;
+  double mx = 0.0;
+  for (int t = threadIdx.x; t < rows * cols; t += 256) mx = fmax(mx, fabs(src[t]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int e = (Form<MODE>::kScaled && red[0] > 0.0) ? 13 - ilogb(red[0]) : 0;
+    e_sh = e;
+    lexp[2 * m + par] = e;
+  }
+  __syncthreads();
+  const int e = e_sh;
+  for (int t = threadIdx.x; t < rows * cols; t += 256) {
+    const int r = t / cols, c = t % cols;
+    const double x = Form<MODE>::kScaled ? ldexp(src[t], e) : src[t];
+    uint16_t pc[3];
+    split<MODE>(x, pc);
+    // transpose: destination row = source column
+    const size_t d = (size_t)c * dcols + r;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) dst[(size_t)q * drows * dcols + d] = pc[q];
+  }
+}
+
+template <int MODE>
+int build_mixed_mode(Plan &p) {
+  constexpr int NP = Form<MODE>::NP;
+  const int N = p.N;
+  const int P2p32 = roundup(p.P2p, 32);
+  std::vector<int> li(N + 1), lf(N + 1);
+  size_t ti = 0, tf = 0;
+  for (int m = 0; m <= N; ++m) {
+    const int ke = ceil_half(N - m + 1);
+    li[m] = (int)ti;
+    lf[m] = (int)tf;
+    ti += (size_t)2 * NP * p.P2p * roundup(ke, 32);
+    tf += (size_t)2 * NP * roundup(ke, 16) * P2p32;
+  }
+  if (ti > (size_t)INT32_MAX || tf > (size_t)INT32_MAX)
+    return fail(SHT_ERR_UNSUPPORTED, "mixed-precision table exceeds 2^31 entries");
+  p.n_linv = ti;
+  p.n_lfwd = tf;
+  SHT_HIP(hipMalloc(&p.Linv, sizeof(uint16_t) * ti));
+  SHT_HIP(hipMalloc(&p.Lfwd, sizeof(uint16_t) * tf));
+  SHT_HIP(hipMalloc(&p.loff_inv, sizeof(int) * (N + 1)));
+  SHT_HIP(hipMalloc(&p.loff_fwd, sizeof(int) * (N + 1)));
+  SHT_HIP(hipMalloc(&p.lexp_inv, sizeof(int) * 2 * (N + 1)));
+  SHT_HIP(hipMalloc(&p.lexp_fwd, sizeof(int) * 2 * (N + 1)));
+  SHT_HIP(hipMemcpy(p.loff_inv, li.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+  SHT_HIP(hipMemcpy(p.loff_fwd, lf.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
+  SHT_HIP(hipMemset(p.Linv, 0, sizeof(uint16_t) * ti));
+  SHT_HIP(hipMemset(p.Lfwd, 0, sizeof(uint16_t) * tf));
+  split_table_kernel<MODE><<<2 * (N + 1), 256>>>(p.Pinv, p.poff_inv, static_cast<uint16_t *>(p.Linv),
+                                                 p.loff_inv, p.lexp_inv, N, p.P2p, P2p32, 1);
+  SHT_HIP(hipGetLastError());
+  split_table_kernel<MODE><<<2 * (N + 1), 256>>>(p.Pfwd, p.poff_fwd, static_cast<uint16_t *>(p.Lfwd),
+                                                 p.loff_fwd, p.lexp_fwd, N, p.P2p, P2p32, 0);
+  SHT_HIP(hipGetLastError());
+  SHT_HIP(hipDeviceSynchronize());
+  return SHT_OK;
+}
+
+template <int MODE, int NSL>
+int inv_nsl(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  leg_inv_mixed_kernel<MODE, NSL><<<(p.N + 1) * nct, 256, 0, s>>>(
+      spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.N, p.P2, p.P2p,
+      ncol, nct, four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+template <int MODE>
+int inv_mode(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+  switch (roundup(ceil_half(p.N + 1), 32) / 32) {
+    case 1: return inv_nsl<MODE, 1>(p, spec, four, nf, s);
+    case 2: return inv_nsl<MODE, 2>(p, spec, four, nf, s);
+    case 3: return inv_nsl<MODE, 3>(p, spec, four, nf, s);
+    case 4: return inv_nsl<MODE, 4>(p, spec, four, nf, s);
+  }
+  return fail(SHT_ERR_UNSUPPORTED, "mixed inverse: nmax above 255");
+}
+
+template <int MODE, int NSL>
+int fwd_nsl(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  leg_fwd_mixed_kernel<MODE, NSL><<<(p.N + 1) * nct, 256, 0, s>>>(
+      four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.N, p.P2,
+      roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+template <int MODE>
+int fwd_mode(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+  switch (roundup(p.P2p, 32) / 32) {
+    case 1: return fwd_nsl<MODE, 1>(p, four, spec, nf, s);
+    case 2: return fwd_nsl<MODE, 2>(p, four, spec, nf, s);
+    case 3: return fwd_nsl<MODE, 3>(p, four, spec, nf, s);
+    case 4: return fwd_nsl<MODE, 4>(p, four, spec, nf, s);
+    case 5: return fwd_nsl<MODE, 5>(p, four, spec, nf, s);
+    case 6: return fwd_nsl<MODE, 6>(p, four, spec, nf, s);
+    case 7: return fwd_nsl<MODE, 7>(p, four, spec, nf, s);
+    case 8: return fwd_nsl<MODE, 8>(p, four, spec, nf, s);
+  }
+  return fail(SHT_ERR_UNSUPPORTED, "mixed forward: jm above 512");
+}
+
+}  // namespace
+
+int build_mixed_tables(Plan &p) {
+  switch (p.prec) {
+    case SHT_PREC_SPLIT_BF16: return build_mixed_mode<SHT_PREC_SPLIT_BF16>(p);
+    case SHT_PREC_SCALED_FP16: return build_mixed_mode<SHT_PREC_SCALED_FP16>(p);
+    case SHT_PREC_SPLIT_BF16X6: return build_mixed_mode<SHT_PREC_SPLIT_BF16X6>(p);
+  }
+  return fail(SHT_ERR_INVALID, "build_mixed_tables: not a mixed-precision plan");
+}
+
+int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+  switch (p.prec) {
+    case SHT_PREC_SPLIT_BF16: return inv_mode<SHT_PREC_SPLIT_BF16>(p, spec, four, nf, s);
+    case SHT_PREC_SCALED_FP16: return inv_mode<SHT_PREC_SCALED_FP16>(p, spec, four, nf, s);
+    case SHT_PREC_SPLIT_BF16X6: return inv_mode<SHT_PREC_SPLIT_BF16X6>(p, spec, four, nf, s);
+  }
+  return fail(SHT_ERR_INVALID, "launch_leg_inv_mixed: not a mixed-precision plan");
+}
+
+int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+  switch (p.prec) {
+    case SHT_PREC_SPLIT_BF16: return fwd_mode<SHT_PREC_SPLIT_BF16>(p, four, spec, nf, s);
+    case SHT_PREC_SCALED_FP16: return fwd_mode<SHT_PREC_SCALED_FP16>(p, four, spec, nf, s);
+    case SHT_PREC_SPLIT_BF16X6: return fwd_mode<SHT_PREC_SPLIT_BF16X6>(p, four, spec, nf, s);
+  }
+  return fail(SHT_ERR_INVALID, "launch_leg_fwd_mixed: not a mixed-precision plan");
 }
 
 }  // namespace sht

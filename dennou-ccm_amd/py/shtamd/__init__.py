@@ -24,7 +24,7 @@ SHT_ERR_DEVICE = -3
 SHT_ERR_UNSUPPORTED = -4
 SHT_ERR_BUSY = -5
 
-PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16 = 0, 1, 2
+PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16, PREC_SPLIT_BF16X6 = 0, 1, 2, 3
 FLAG_GRAD = 1
 
 _lib = None

@@ -54,12 +54,14 @@ enum {
   SHT_ERR_BUSY = -5         /* request server: no memory or slot free for admission */
 };
 
-/* Arithmetic of the Legendre step.  The FFT, the tables and the final
- * spectral accumulation are FP64 in every form. */
+/* Arithmetic of the Legendre step.  The FFT, the table generation and the
+ * accumulation across 32-wide K steps are FP64 in every form; the mixed forms
+ * keep the Fourier intermediate in FP32. */
 enum {
-  SHT_PREC_FP64 = 0,        /* FP64 MFMA (v_mfma_f64_16x16x4) */
-  SHT_PREC_SPLIT_BF16 = 1,  /* BF16 hi/lo splits, three products, FP32 accumulate */
-  SHT_PREC_SCALED_FP16 = 2  /* per-m column-scaled FP16 hi/lo, FP32 accumulate */
+  SHT_PREC_FP64 = 0,         /* FP64 MFMA (v_mfma_f64_16x16x4) */
+  SHT_PREC_SPLIT_BF16 = 1,   /* BF16 hi/lo pieces, three products, FP32 step sums */
+  SHT_PREC_SCALED_FP16 = 2,  /* FP16 hi/lo pieces scaled per m and per column, three products */
+  SHT_PREC_SPLIT_BF16X6 = 3  /* BF16 hi/mid/lo pieces, six products */
 };
 
 enum { SHT_FLAG_GRAD = 1 }; /* also build the derivative table (GradLat) */
