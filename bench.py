@@ -190,6 +190,67 @@ def traffic_from_profile(form: str, kernel_prefix: str):
 
 KERNEL_NAMES = {"fft_inv": "fft_inv_kernel", "fft_fwd": "fft_fwd_kernel"}
 
+ENSEMBLE_GRIDS = {42: (128, 64), 85: (256, 128), 170: (512, 256)}
+
+
+def run_ensemble(precision: str, nmember: int, steps: int, warmup: int, seed: int = 20261202):
+    """Ensemble of nmember members, truncation uniform in {T42, T85, T170},
+    levels uniform in {16, 32, 48} (3L+1 fields each), one graph launch per
+    round trip; returns a JSON-ready dict."""
+    import torch
+
+    import oracle
+    import shtamd
+    from shtamd import hipmem, synth
+
+    rng = np.random.default_rng(seed)
+    Ts = rng.choice([42, 85, 170], size=nmember)
+    Ls = rng.choice([16, 32, 48], size=nmember)
+    members = [(int(T), *ENSEMBLE_GRIDS[int(T)], 3 * int(L) + 1) for T, L in zip(Ts, Ls)]
+    b = shtamd.Batch(members, precision=PRECISIONS[precision])
+    specs = []
+    for j, (N, im, jm, nf) in enumerate(members):
+        s = synth.state_spectra(N, (nf - 1) // 3, cfg=3, member=j)
+        spec_in, _, _, ld = b.member_ptrs(j)
+        hipmem.put_member_spectra(spec_in, ld, s)
+        specs.append(s)
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, warmup)):
+        b.run(shtamd.DIR_ROUNDTRIP, stream)
+    torch.cuda.synchronize()
+    times = {}
+    for mode in ("graph", "eager"):
+        fn = b.run if mode == "graph" else b.run_eager
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(SPIN_CYCLES)
+        e0.record(stream)
+        for _ in range(steps):
+            fn(shtamd.DIR_ROUNDTRIP, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times[mode] = e0.elapsed_time(e1) / steps
+    # per-member error against the FP64 oracle
+    errs = []
+    plans = {}
+    for j, (N, im, jm, nf) in enumerate(members):
+        spec_in, grid, spec_out, ld = b.member_ptrs(j)
+        K = (N + 1) * (N + 2) // 2
+        g = hipmem.from_device(grid, (nf, jm, im))
+        s_rt = hipmem.get_member_spectra(spec_out, ld, K, nf)
+        ref = plans.setdefault(N, oracle.Plan(N, im, jm))
+        errs.append(max(rel(g, ref.spec_to_grid(specs[j])), rel(s_rt, specs[j])))
+    b.close()
+    nf_total = sum(m[3] for m in members)
+    best = min(times.values())
+    return {
+        "metric": "ensemble SHT fields/s (64 members, T42/T85/T170 x L16/32/48, one graph launch)",
+        "value": round(nf_total / (best * 1e-3), 1), "unit": "fields/s", "precision": precision,
+        "members": nmember, "fields": nf_total,
+        "members_per_truncation": {f"T{t}": int((Ts == t).sum()) for t in (42, 85, 170)},
+        "ms_per_round_trip": {k: round(v, 4) for k, v in times.items()},
+        "max_member_rel_l2_vs_fp64_oracle": max(errs),
+    }
+
 
 def main():
     ap = argparse.ArgumentParser()
@@ -199,6 +260,9 @@ def main():
     ap.add_argument("--precision", default="auto", choices=sorted(PRECISIONS) + ["auto"],
                     help="auto: run every form; the headline is split-BF16 (six products, <=1e-6)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble"],
+                    help="ensemble: the 64-member mixed-truncation batch (separate JSON line)")
+    ap.add_argument("--members", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -206,6 +270,11 @@ def main():
         sys.exit(2)
 
     import oracle
+
+    if args.workload == "ensemble":
+        prec = "bf16x6" if args.precision == "auto" else args.precision
+        print(json.dumps(run_ensemble(prec, args.members, max(1, args.steps // 10), args.warmup)))
+        return
 
     forms = list(PRECISIONS) if args.precision == "auto" else sorted({"fp64", args.precision})
     headline = "bf16x6" if args.precision == "auto" else args.precision

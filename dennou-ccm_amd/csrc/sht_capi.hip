@@ -138,6 +138,17 @@ static int spec_to_grad(Plan &p, const double *spec, double *grid, double *glon,
   });
 }
 
+// One direction of the transform on a plan (used by the ensemble batches).
+int run_dir_on(sht_plan *plan, const double *spec_in, double *grid, double *spec_out, int nf,
+               int dir, hipStream_t s) {
+  Plan &p = plan->p;
+  int rc = SHT_OK;
+  if (dir == SHT_DIR_INVERSE || dir == SHT_DIR_ROUNDTRIP) rc = spec_to_grid(p, spec_in, grid, nf, s);
+  if (!rc && (dir == SHT_DIR_FORWARD || dir == SHT_DIR_ROUNDTRIP))
+    rc = grid_to_spec(p, grid, spec_out, nf, s);
+  return rc;
+}
+
 }  // namespace sht
 
 using namespace sht;

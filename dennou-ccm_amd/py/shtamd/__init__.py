@@ -25,6 +25,7 @@ SHT_ERR_UNSUPPORTED = -4
 SHT_ERR_BUSY = -5
 
 PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16, PREC_SPLIT_BF16X6 = 0, 1, 2, 3
+DIR_INVERSE, DIR_FORWARD, DIR_ROUNDTRIP = 0, 1, 2
 FLAG_GRAD = 1
 
 _lib = None
@@ -39,6 +40,10 @@ class ShtError(RuntimeError):
 class _Desc(ctypes.Structure):
     _fields_ = [("nmax", ctypes.c_int), ("im", ctypes.c_int), ("jm", ctypes.c_int),
                 ("kmax", ctypes.c_int), ("precision", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+class _MemberDesc(ctypes.Structure):
+    _fields_ = [("nmax", ctypes.c_int), ("im", ctypes.c_int), ("jm", ctypes.c_int), ("nfield", ctypes.c_int)]
 
 
 _vp = ctypes.c_void_p
@@ -64,6 +69,12 @@ SIGNATURES = {
     "sht_wa_xya": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_xya_gradlon_wa": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_xya_gradlat_wa": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_batch_create": (_ci, [ctypes.POINTER(_MemberDesc), _ci, _ci, ctypes.POINTER(_vp)]),
+    "sht_batch_destroy": (None, [_vp]),
+    "sht_batch_member": (_ci, [_vp, _ci, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                               ctypes.POINTER(_ci)]),
+    "sht_batch_run": (_ci, [_vp, _ci, _vp]),
+    "sht_batch_run_eager": (_ci, [_vp, _ci, _vp]),
     "sht_plan_set_timing": (_ci, [_vp, _ci]),
     "sht_plan_stage_times": (_ci, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 }
@@ -197,6 +208,44 @@ class Plan:
 
     def xya_gradlat_wa(self, wa):
         return self._host_call(lib().sht_xya_gradlat_wa, wa, (wa.shape[0], self.jm, self.im))
+
+
+class Batch:
+    """Ensemble batch (sht_batch_*): members of uneven truncation and level
+    count, transformed by one graph launch.  members: list of
+    (nmax, im, jm, nfield)."""
+
+    def __init__(self, members, precision: int = PREC_FP64):
+        arr = (_MemberDesc * len(members))(*[_MemberDesc(*m) for m in members])
+        h = _vp()
+        check(lib().sht_batch_create(arr, len(members), precision, ctypes.byref(h)))
+        self._h = h
+        self.members = [tuple(m) for m in members]
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().sht_batch_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def member_ptrs(self, j: int):
+        """(spec_in, grid, spec_out, ld) device addresses of member j."""
+        a, b, c, ld = _vp(), _vp(), _vp(), _ci()
+        check(lib().sht_batch_member(self._h, j, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                                     ctypes.byref(ld)))
+        return a.value, b.value, c.value, ld.value
+
+    def run(self, direction: int = DIR_ROUNDTRIP, stream=None):
+        check(lib().sht_batch_run(self._h, direction, _stream_handle(stream)))
+
+    def run_eager(self, direction: int = DIR_ROUNDTRIP, stream=None):
+        check(lib().sht_batch_run_eager(self._h, direction, _stream_handle(stream)))
 
 
 def spec_to_real(spec_c: np.ndarray) -> np.ndarray:

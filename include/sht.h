@@ -126,6 +126,37 @@ int sht_wa_xya(sht_plan *plan, const double *xya, double *wa, int nlev);
 int sht_xya_gradlon_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
 int sht_xya_gradlat_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
 
+/* ---- ensemble batches: many members of uneven truncation and level count
+ *      transformed by one hipGraph launch.  There is no SPML counterpart (SPML
+ *      holds one global plan); this is the batched form of wa_xya/xya_wa for
+ *      ensemble runs.  Members are grouped by grid (nmax, im, jm); each group's
+ *      members are consecutive field ranges of one resident batch, so every
+ *      kernel launch covers all members of a group, and the groups run as
+ *      independent branches of the graph.                                    */
+typedef struct sht_batch sht_batch;
+typedef struct {
+  int nmax, im, jm; /* truncation and grid of the member */
+  int nfield;       /* 2-D fields of the member (levels x variables)        */
+} sht_member_desc;
+enum { SHT_DIR_INVERSE = 0, SHT_DIR_FORWARD = 1, SHT_DIR_ROUNDTRIP = 2 };
+/* Allocates the member state in HBM (canonical spectra in, grid, spectra out)
+ * and one plan per grid with the given precision. */
+int sht_batch_create(const sht_member_desc *members, int nmember, int precision,
+                     sht_batch **batch);
+void sht_batch_destroy(sht_batch *batch);
+/* Device views of member j.  Spectra: complex s_n^m of field f at
+ * spec[(k * ld + 2 f) + {0,1}], k = off(m) + n - m (ld = row stride in doubles;
+ * the member's fields are columns of its group's batch).  Grid: [f][j][i].
+ * Any output pointer may be NULL. */
+int sht_batch_member(const sht_batch *batch, int j, double **spec_in, double **grid,
+                     double **spec_out, int *ld);
+/* Runs the direction (INVERSE: spec_in -> grid; FORWARD: grid -> spec_out;
+ * ROUNDTRIP: both) for every member with one graph launch on stream.  The
+ * graph is captured on the first call of each direction. */
+int sht_batch_run(sht_batch *batch, int direction, void *stream);
+/* The same work launched kernel by kernel (no graph), for comparison. */
+int sht_batch_run_eager(sht_batch *batch, int direction, void *stream);
+
 /* ---- stage timers: HIP events around every kernel launch of a plan, summed
  *      per stage -- the counterpart of DCPAM's ProfUtil_RapStart/RapEnd
  *      regions around the dynamical core (atm/dcpam_main_mod.f90:2167, :2203).
