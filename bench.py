@@ -171,6 +171,48 @@ def cpu_baseline(spec_c, seconds: float, threads: int):
     return n * NF / dt, n, dt
 
 
+def run_serve(precision: str, rates, nreq_per_rate, slots: int = 4, max_fields: int = 256,
+              err_sample: int = 100):
+    """Serving workload: Poisson request streams; returns a JSON-ready dict."""
+    import torch
+
+    import oracle
+    import shtamd
+    from shtamd import serve
+
+    grids = [(T, *serve.GRIDS[T]) for T in (21, 42, 85, 170)]
+    out = {"precision": precision, "slots": slots, "max_fields": max_fields, "rates": {}}
+    for rate in rates:
+        n = int(nreq_per_rate(rate))
+        reqs = serve.make_requests(n, seed=int(rate))
+        data = serve.build(torch, reqs)
+        srv = shtamd.Server(grids, slots=slots, max_fields=max_fields, precision=PRECISIONS[precision])
+        lat, wall, retries = serve.run_stream(torch, srv, data, rate)
+        srv.drain()
+        st = srv.stats()
+        srv.close()
+        fields = sum(L for _, L, _ in reqs)
+        # error against the FP64 oracle on a sample of the requests
+        rng = np.random.default_rng(1)
+        errs = []
+        plans = {}
+        for k in rng.choice(n, size=min(err_sample, n), replace=False):
+            d = data[k]
+            ref = plans.setdefault(d.T, oracle.Plan(d.T, *serve.GRIDS[d.T]))
+            if d.direction == shtamd.DIR_INVERSE:
+                errs.append(rel(d.out.cpu().numpy(), ref.spec_to_grid(d.spec)))
+            else:
+                errs.append(rel(shtamd.real_to_spec(d.out.cpu().numpy()), ref.grid_to_spec(d.inp.cpu().numpy())))
+        out["rates"][str(rate)] = {
+            "requests": n, "p50_ms": round(float(np.percentile(lat, 50)), 4),
+            "p99_ms": round(float(np.percentile(lat, 99)), 4), "max_ms": round(float(np.max(lat)), 4),
+            "fields_per_s": round(fields / wall, 1), "batches": st["batches"], "refused_retries": retries,
+            "max_rel_l2_vs_fp64_oracle": max(errs)}
+        del data
+        torch.cuda.empty_cache()
+    return out
+
+
 def traffic_from_profile(form: str, kernel_prefix: str):
     """HBM bytes per dispatch of a kernel from the committed rocprofv3 summary
     profiles/t85l32_<form>_hbm_traffic.md (FETCH_SIZE/WRITE_SIZE runs scaled by a
@@ -260,8 +302,11 @@ def main():
     ap.add_argument("--precision", default="auto", choices=sorted(PRECISIONS) + ["auto"],
                     help="auto: run every form; the headline is split-BF16 (six products, <=1e-6)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble"],
-                    help="ensemble: the 64-member mixed-truncation batch (separate JSON line)")
+    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve"],
+                    help="ensemble: the 64-member mixed-truncation batch; serve: request streams at "
+                         "100-1000 req/s (separate JSON lines)")
+    ap.add_argument("--serve-requests", type=int, default=500,
+                    help="requests in the 500 req/s serving run of the headline line (0 = skip)")
     ap.add_argument("--members", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -274,6 +319,12 @@ def main():
     if args.workload == "ensemble":
         prec = "bf16x6" if args.precision == "auto" else args.precision
         print(json.dumps(run_ensemble(prec, args.members, max(1, args.steps // 10), args.warmup)))
+        return
+    if args.workload == "serve":
+        prec = "bf16x6" if args.precision == "auto" else args.precision
+        res = run_serve(prec, [100, 250, 500, 1000], lambda r: max(200, min(2000, 2 * r)))
+        res["metric"] = "SHT serving p50/p99 latency (ms) under Poisson load, T21/T42/T85/T170 mix"
+        print(json.dumps(res))
         return
 
     forms = list(PRECISIONS) if args.precision == "auto" else sorted({"fp64", args.precision})
@@ -323,6 +374,10 @@ def main():
         per_form[form] = t
     fp64_v = per_form["fp64"]["fields_per_s"]
 
+    serve_res = None
+    if args.serve_requests > 0:
+        serve_res = run_serve(headline, [500], lambda r: args.serve_requests)["rates"]["500"]
+
     threads = min(16, os.cpu_count() or 1)
     cpu_v, cpu_n, cpu_dt = cpu_baseline(res["spec_in"], args.cpu_seconds, threads)
     cpu1_v, cpu1_n, cpu1_dt = cpu_baseline(res["spec_in"], max(2.0, args.cpu_seconds / 4), 1)
@@ -348,6 +403,7 @@ def main():
         "err_vs_fp64_oracle": res["err"],
         "speedup_vs_fp64_form": round(value / fp64_v, 3),
         "forms": per_form,
+        "serve_500rps": serve_res,
         "roofline": roofline,
         "cpu_baseline": {"value": round(cpu_v, 1), "unit": "fields/s", "cores": threads, "kind": "port",
                          "sample": f"oracle round trips of the same 97-field T85L32 batch, {cpu_n} reps "

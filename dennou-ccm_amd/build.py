@@ -25,7 +25,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}",
 ]
 SOURCES = ["sht_capi.hip", "sht_tables.hip", "sht_fft.hip", "sht_legendre.hip",
-           "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip"]
+           "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip", "sht_server.hip"]
 
 
 def _compile(src: str) -> str:
@@ -43,7 +43,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

@@ -46,6 +46,17 @@ class _MemberDesc(ctypes.Structure):
     _fields_ = [("nmax", ctypes.c_int), ("im", ctypes.c_int), ("jm", ctypes.c_int), ("nfield", ctypes.c_int)]
 
 
+class _ServerDesc(ctypes.Structure):
+    _fields_ = [("ngrid", ctypes.c_int), ("nmax", ctypes.c_int * 8), ("im", ctypes.c_int * 8),
+                ("jm", ctypes.c_int * 8), ("slots", ctypes.c_int), ("max_fields", ctypes.c_int),
+                ("precision", ctypes.c_int), ("mem_budget", ctypes.c_size_t)]
+
+
+class _Request(ctypes.Structure):
+    _fields_ = [("nmax", ctypes.c_int), ("nfield", ctypes.c_int), ("direction", ctypes.c_int),
+                ("inp", ctypes.c_void_p), ("out", ctypes.c_void_p)]
+
+
 _vp = ctypes.c_void_p
 _ci = ctypes.c_int
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -75,6 +86,12 @@ SIGNATURES = {
                                ctypes.POINTER(_ci)]),
     "sht_batch_run": (_ci, [_vp, _ci, _vp]),
     "sht_batch_run_eager": (_ci, [_vp, _ci, _vp]),
+    "sht_server_create": (_ci, [ctypes.POINTER(_ServerDesc), ctypes.POINTER(_vp)]),
+    "sht_server_destroy": (None, [_vp]),
+    "sht_server_submit": (_ci, [_vp, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_longlong)]),
+    "sht_server_poll": (_ci, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp, _ci, ctypes.POINTER(_ci)]),
+    "sht_server_drain": (_ci, [_vp]),
+    "sht_server_stats": (_ci, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "sht_plan_set_timing": (_ci, [_vp, _ci]),
     "sht_plan_stage_times": (_ci, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
 }
@@ -246,6 +263,55 @@ class Batch:
 
     def run_eager(self, direction: int = DIR_ROUNDTRIP, stream=None):
         check(lib().sht_batch_run_eager(self._h, direction, _stream_handle(stream)))
+
+
+class Server:
+    """Request server (sht_server_*).  grids: list of (nmax, im, jm)."""
+
+    def __init__(self, grids, slots: int = 2, max_fields: int = 128, precision: int = PREC_FP64,
+                 mem_budget: int = 8 << 30):
+        d = _ServerDesc()
+        d.ngrid = len(grids)
+        for i, (n, im, jm) in enumerate(grids):
+            d.nmax[i], d.im[i], d.jm[i] = n, im, jm
+        d.slots, d.max_fields, d.precision, d.mem_budget = slots, max_fields, precision, mem_budget
+        h = _vp()
+        check(lib().sht_server_create(ctypes.byref(d), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().sht_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, nmax: int, nfield: int, direction: int, inp: int, out: int) -> int:
+        """Returns the ticket; raises ShtError (code SHT_ERR_BUSY when not admitted)."""
+        r = _Request(nmax, nfield, direction, inp, out)
+        t = ctypes.c_longlong()
+        check(lib().sht_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, max_n: int = 4096):
+        t = (ctypes.c_longlong * max_n)()
+        lat = (ctypes.c_double * max_n)()
+        n = _ci()
+        check(lib().sht_server_poll(self._h, t, lat, max_n, ctypes.byref(n)))
+        return [(t[i], lat[i]) for i in range(n.value)]
+
+    def drain(self):
+        check(lib().sht_server_drain(self._h))
+
+    def stats(self):
+        v = [ctypes.c_longlong() for _ in range(4)]
+        check(lib().sht_server_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("requests", "fields", "batches", "refused"), (x.value for x in v)))
 
 
 def spec_to_real(spec_c: np.ndarray) -> np.ndarray:

@@ -157,6 +157,47 @@ int sht_batch_run(sht_batch *batch, int direction, void *stream);
 /* The same work launched kernel by kernel (no graph), for comparison. */
 int sht_batch_run_eager(sht_batch *batch, int direction, void *stream);
 
+/* ---- request server: continuous batching of transform requests whose data
+ *      are resident in HBM.  No SPML counterpart (SPML is a library called by
+ *      one model); this serves many callers from resident tables.  A worker
+ *      thread owns `slots` HIP streams, each with a plan per grid; whenever a
+ *      slot is free it takes the pending requests of one (grid, direction) in
+ *      arrival order (up to max_fields fields), gathers them into its resident
+ *      batch buffers, transforms them and scatters the results, so requests
+ *      that arrive while others run are admitted into the next free slot and
+ *      their work overlaps on the device.  Submission is admitted against a
+ *      memory budget (bytes of request data pending or in flight) and returns
+ *      SHT_ERR_BUSY when it would exceed it.                                  */
+typedef struct sht_server sht_server;
+typedef struct {
+  int ngrid;             /* grids served (<= 8)                              */
+  int nmax[8], im[8], jm[8];
+  int slots;             /* concurrent batches (streams), 1..8               */
+  int max_fields;        /* fields per batch and per request                 */
+  int precision;         /* SHT_PREC_*                                       */
+  size_t mem_budget;     /* admission budget in bytes                        */
+} sht_server_desc;
+typedef struct {
+  int nmax;              /* selects the grid                                 */
+  int nfield;
+  int direction;         /* SHT_DIR_INVERSE or SHT_DIR_FORWARD               */
+  const double *in;      /* device: canonical spectra [K][nfield] (inverse) or grid */
+  double *out;           /* device: grid (inverse) or canonical spectra      */
+} sht_request;
+int sht_server_create(const sht_server_desc *desc, sht_server **server);
+/* Stops the worker after the requests in flight complete. */
+void sht_server_destroy(sht_server *server);
+/* Non-blocking; *ticket identifies the request.  SHT_ERR_BUSY: not admitted. */
+int sht_server_submit(sht_server *server, const sht_request *req, long long *ticket);
+/* Completed requests since the last call: tickets and submit-to-completion
+ * latencies in milliseconds; *n returns how many (at most max). */
+int sht_server_poll(sht_server *server, long long *tickets, double *latency_ms, int max, int *n);
+/* Blocks until nothing is pending or in flight. */
+int sht_server_drain(sht_server *server);
+/* Counters: completed requests, fields, batches launched, requests refused. */
+int sht_server_stats(const sht_server *server, long long *requests, long long *fields,
+                     long long *batches, long long *refused);
+
 /* ---- stage timers: HIP events around every kernel launch of a plan, summed
  *      per stage -- the counterpart of DCPAM's ProfUtil_RapStart/RapEnd
  *      regions around the dynamical core (atm/dcpam_main_mod.f90:2167, :2203).
