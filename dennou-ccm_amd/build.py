@@ -37,6 +37,34 @@ def _compile(src: str) -> str:
     return out
 
 
+FORTRAN = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FTEST = os.path.join(ROOT, "tests", "fortran", "test_wa_module.f90")
+
+
+def build_fortran(verbose: bool = False) -> str | None:
+    """Compiles the Fortran wa_module (ISO_C_BINDING over the C ABI) into
+    lib/wa_module.o + lib/wa_module.mod and links the drop-in test program
+    lib/test_wa_module against libsht_hip.so.  Skipped without amdflang."""
+    if not os.path.exists(FORTRAN):
+        return None
+    libdir = os.path.dirname(LIB)
+    obj = os.path.join(libdir, "wa_module.o")
+    exe = os.path.join(libdir, "test_wa_module")
+    steps = [
+        [FORTRAN, "-O2", "-fPIC", "-c", os.path.join(FDIR, "wa_module.f90"), "-module-dir", libdir, "-o", obj],
+        [FORTRAN, "-O2", FTEST, obj, "-module-dir", libdir, "-I", libdir, "-L", libdir, "-lsht_hip",
+         "-Wl,-rpath,$ORIGIN", "-o", exe],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=libdir)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stderr}")
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(os.path.dirname(LIB), exist_ok=True)
@@ -50,6 +78,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -62,3 +62,16 @@ def test_null_plan_calls_fail_cleanly():
     assert L.sht_spec_to_grid_dev(None, None, None, 1, None) == shtamd.SHT_ERR_INVALID
     assert L.sht_xya_wa(None, None, None, 1) == shtamd.SHT_ERR_INVALID
     L.sht_plan_destroy(None)
+
+
+def test_fortran_module_compiles_against_the_abi():
+    """The amdflang build of wa_module and its drop-in program links against
+    libsht_hip.so (run on the GPU in test_gpu_fortran.py)."""
+    import os
+    import shutil
+
+    if not shutil.which("amdflang") and not os.path.exists("/opt/rocm/bin/amdflang"):
+        pytest.skip("amdflang not available")
+    libdir = os.path.dirname(shtamd.LIB_PATH)
+    assert os.path.exists(os.path.join(libdir, "wa_module.mod"))
+    assert os.path.exists(os.path.join(libdir, "test_wa_module"))
