@@ -311,7 +311,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
-                                   "(the transform shards by field batch; see DESIGN.md)"}))
+                                   "(the transform would shard by field batch, one slab of levels per GPU)"}))
         sys.exit(2)
 
     import oracle

@@ -6,7 +6,7 @@
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * it.  The product path never links or calls anything here.
  *
- * Conventions (shared with the HIP library, written out once in DESIGN.md):
+ * Conventions (shared with the HIP library, include/sht.h):
  *   grid   g[f][j][i]     f = field (level), j = 0..jm-1 Gauss latitudes S->N,
  *                         i = 0..im-1 longitudes lambda_i = 2*pi*i/im
  *                         (Fortran xya(0:im-1,1:jm,1:k), dcpam_main_mod.f90:4107-4125)
