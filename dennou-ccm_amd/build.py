@@ -25,7 +25,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-result", f"-I{os.path.join(ROOT, 'include')}",
 ]
 SOURCES = ["sht_capi.hip", "sht_tables.hip", "sht_fft.hip", "sht_legendre.hip",
-           "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip", "sht_server.hip"]
+           "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip", "sht_server.hip",
+           "sht_tune.hip"]
 
 
 def _compile(src: str) -> str:

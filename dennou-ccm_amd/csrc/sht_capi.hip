@@ -46,7 +46,7 @@ static void release(Plan &p) {
   for (hipEvent_t e : p.event_pool) (void)hipEventDestroy(e);
   void *ptrs[] = {p.mu, p.w, p.coslat, p.twid, p.Pinv, p.Hinv, p.Pfwd, p.poff_inv, p.poff_fwd,
                   p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.lexp_inv, p.lexp_fwd, p.four, p.four2,
-                  p.stage_a, p.stage_b};
+                  p.stage_a, p.stage_b, p.fwd_units, p.zero, p.xcd_m[0], p.xcd_m[1]};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p.pinned) (void)hipHostFree(p.pinned);

@@ -57,6 +57,7 @@ struct Plan {
   // device tables
   double *mu = nullptr, *w = nullptr, *coslat = nullptr; // jm each, S->N
   double *twid = nullptr;                                // im complex: e^{2 pi i t/im}
+  double *zero = nullptr;   // 64 zeros: the source of masked-off operand loads
   double *Pinv = nullptr, *Hinv = nullptr, *Pfwd = nullptr;
   int *poff_inv = nullptr, *poff_fwd = nullptr;          // N+1 offsets (doubles)
   std::vector<int> h_poff_inv, h_poff_fwd;
@@ -66,6 +67,19 @@ struct Plan {
   int *loff_inv = nullptr, *loff_fwd = nullptr;
   int *lexp_inv = nullptr, *lexp_fwd = nullptr; // FP16 power-of-two per (m, parity)
   size_t n_linv = 0, n_lfwd = 0;
+  // wave-tile forward Legendre: (m << 8 | 16-degree row tile), heaviest m first
+  int *fwd_units = nullptr;
+  int n_fwd_units = 0;
+  // Wavenumbers dealt to the 8 XCDs, [xcd][mlen] with -1 padding, balanced by
+  // the work of each m (greedy, heaviest first): one list for the inverse
+  // (work ~ K steps), one for the forward (work ~ 16-degree row tiles).
+  int *xcd_m[2] = {nullptr, nullptr};
+  int xcd_mlen[2] = {0, 0};
+  // Legendre kernel shapes (SHT_SHAPE_*): the default for every field count,
+  // and entries fixed by sht_plan_set_shapes / sht_plan_autotune for one count
+  int shape_inv = SHT_SHAPE_AUTO, shape_fwd = SHT_SHAPE_AUTO;
+  struct ShapeEntry { int nf, inv, fwd; };
+  std::vector<ShapeEntry> shapes;
   // scratch
   double *four = nullptr, *four2 = nullptr; // [P2][N+1][2][2*kmax]
   double *stage_a = nullptr, *stage_b = nullptr; // device staging for host calls
@@ -113,7 +127,13 @@ int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int 
 int launch_fft_inv(const Plan &p, const void *four, const void *four2, bool f32, double *grid,
                    double *gradlon, double *gradlat, int nf, hipStream_t s);
 
-// Legendre stage, FP64 MFMA (sht_legendre.hip)
+// Legendre stage, FP64 MFMA (sht_legendre.hip).  Shapes resolve through
+// resolve_shape: an entry for nf, else the plan default, else the built-in one.
+int resolve_shape(const Plan &p, int nf, int dir);  // dir 0 inverse, 1 forward
+int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, double *four,
+                             double *four2, bool value_table, bool deriv_table, int nf, hipStream_t s);
+int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
+                             hipStream_t s);
 int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
                        bool value_table, bool deriv_table, int nf, hipStream_t s);
 int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s);
@@ -128,9 +148,32 @@ int launch_spml_to_canon(const Plan &p, const double *wa, double *spec, int nf, 
 int launch_canon_to_spml(const Plan &p, const double *spec, double *wa, int nf, hipStream_t s);
 int launch_avr_lonlat(const Plan &p, const double *grid, double *avg, int nf, hipStream_t s);
 
+// Index of the calling wavefront in its workgroup, in a scalar register: the
+// compiler then knows that everything derived from it is uniform across the
+// wave, so loops and branches on it stay scalar (no exec-mask divergence and
+// no per-iteration memory waits).
+__device__ __forceinline__ int wave_id() {
+  return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+}
+
 // XCD-aware block order: the hardware deals consecutive workgroup ids to the 8
 // XCDs in turn; this maps them so that each XCD receives a contiguous range of
 // logical tiles (neighbouring tiles share table rows in that XCD's L2).
+// Balanced, L2-local dealing of whole wavenumbers: block b runs on XCD b % 8;
+// that XCD's (b / 8)-th block is tile t of the j-th m of its list, with
+// tiles_per_m blocks per m.  Returns false for the padding blocks.
+__device__ __forceinline__ bool xcd_m_block(int b, const int *__restrict__ mlist, int mlen,
+                                            int tiles_per_m, int &m, int &t) {
+  const int x = b & 7, i = b >> 3, j = i / tiles_per_m;
+  t = i - j * tiles_per_m;
+  if (j >= mlen) return false;
+  m = mlist[x * mlen + j];
+  return m >= 0;
+}
+inline int xcd_m_blocks(const Plan &p, int dir, int tiles_per_m) {
+  return 8 * p.xcd_mlen[dir] * tiles_per_m;
+}
+
 __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
   const int nx = 8;
   int xcd = b % nx, idx = b / nx;

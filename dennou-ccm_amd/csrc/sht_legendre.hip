@@ -16,7 +16,10 @@
 // is row 4v + l/16, column l%16 (measured on gfx950 with a one-hot lane
 // probe).  All global fragments are 128-byte runs.
 // Work per m shrinks with m (N - m + 1 degrees), so tiles are ordered heaviest
-// first and dealt to XCDs in contiguous ranges (xcd_tile).
+// first and dealt to the XCDs round robin in launch order (consecutive
+// workgroups go to consecutive XCDs), which gives every XCD an equal share of
+// heavy and light m; contiguous ranges per XCD would give the first XCD twice
+// the average work.
 #include "sht_internal.h"
 
 namespace sht {
@@ -42,16 +45,15 @@ template <int RT>
 __global__ __launch_bounds__(256) void leg_inv_f64_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
     const double *__restrict__ Pinv, const double *__restrict__ Hinv,
-    const int *__restrict__ poff, int N, int P2, int P2p, int ncol, int nct, int tab0, int ntab,
-    FourGeom geo) {
+    const int *__restrict__ poff, const int *__restrict__ mlist, int mlen, int N, int P2, int P2p,
+    int ncol, int nct, int tab0, int ntab, FourGeom geo) {
   extern __shared__ double red[]; // [kWaves][RT][4][64]
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int ct = b % nct;
-  const int rest = b / nct;
-  const int tab = tab0 + rest % ntab;
-  const int m = rest / ntab;
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, ntab * nct, m, t)) return;
+  const int ct = t % nct;
+  const int tab = tab0 + t / nct;
   const int M1 = N + 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP = (ke + 3) & ~3, nsteps = KP >> 2;
   const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m];
@@ -119,14 +121,15 @@ template <int MAXRT>
 __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restrict__ four,
                                                           double *__restrict__ spec,
                                                           const double *__restrict__ Pfwd,
-                                                          const int *__restrict__ poff, int N,
-                                                          int P2, int P2p, int ncol, int nct,
+                                                          const int *__restrict__ poff,
+                                                          const int *__restrict__ mlist, int mlen,
+                                                          int N, int P2, int P2p, int ncol, int nct,
                                                           FourGeom geo) {
   extern __shared__ double red[]; // [kWaves][MAXRT][4][64]
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int ct = b % nct, m = b / nct;
+  int m, ct;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
   const int M1 = N + 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
   const int nsteps = P2p >> 2;
@@ -187,13 +190,482 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
   }
 }
 
+// Wave-tile shape: one wavefront owns one 16 x 16 output tile for both parities
+// and runs the whole K loop itself, so there is no cross-wave reduction, no
+// LDS and no barrier, and a wave retires as soon as its tile is stored.  The
+// K loop is software-pipelined over chunks of D steps (4 K values each): the
+// fragments of chunk i+1 are in flight while chunk i runs on the matrix core.
+// Loads go to clamped, always-valid addresses and the B operand is masked, so
+// every chunk is straight-line code and the compiler keeps a running wait count.
+// Units are numbered with the column tile fastest, so the four waves of a
+// workgroup share table fragments (same m and row tile) in L1.
+
+// Inverse, unit = (m, table, row tile of 16 pairs, column tile); heaviest m first.
+template <int D>
+__global__ __launch_bounds__(256) void leg_inv_f64_wave_kernel(
+    const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
+    const double *__restrict__ Pinv, const double *__restrict__ Hinv,
+    const int *__restrict__ poff, const int *__restrict__ mlist, int mlen, int N, int P2, int P2p,
+    int ncol, int nct, int tab0, int ntab, FourGeom geo) {
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int rtiles = P2p >> 4;
+  const int upm = rtiles * ntab * nct;  // waves per m
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (upm + 3) >> 2, m, t)) return;
+  const int u = t * 4 + wave;
+  if (u >= upm) return;
+  const int ct = u % nct;
+  const int rest = u / nct;
+  const int rt = rest % rtiles;
+  const int tab = tab0 + rest / rtiles;
+  const int M1 = N + 1;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP = (ke + 3) & ~3, nsteps = KP >> 2;
+  const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m] + rt * 16 + r16;
+  double *__restrict__ out = tab ? four2 : four;
+  const int se = tab ? 1 : 0, so = 1 - se;
+  const int col = ct * 16 + r16;
+  const bool cok = col < ncol;
+  const double *__restrict__ sp = spec + (size_t)coef_off(N, m) * ncol + (cok ? col : 0);
+
+  double a[2][D][2], b[2][D][2];
+  auto load = [&](int buf, int s0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int st = s0 + j, k = st * 4 + kq;
+      const int kc = st < nsteps ? k : kq;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bool vb = cok && k < (q ? ko : ke);
+        a[buf][j][q] = T[(size_t)(q * KP + kc) * P2p];
+        const double x = sp[(size_t)(vb ? 2 * k + q : 0) * ncol];
+        b[buf][j][q] = vb ? x : 0.0;
+      }
+    }
+  };
+  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  auto run = [&](int buf, int s0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (s0 + j < nsteps) {
+        acc0 = mfma_f64(a[buf][j][0], b[buf][j][0], acc0);
+        acc1 = mfma_f64(a[buf][j][1], b[buf][j][1], acc1);
+      }
+  };
+  load(0, 0);
+  for (int s0 = 0; s0 < nsteps; s0 += 2 * D) {
+    load(1, s0 + D);
+    run(0, s0);
+    if (s0 + D >= nsteps) break;
+    load(0, s0 + 2 * D);
+    run(1, s0 + D);
+  }
+  if (cok) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int p = rt * 16 + i * 4 + kq;
+      if (p < P2) {
+        out[geo.at(p, m, se, col)] = acc0[i];
+        out[geo.at(p, m, so, col)] = acc1[i];
+      }
+    }
+  }
+}
+
+// Forward, unit = (m, 16-degree row tile, column tile) from the plan's unit
+// list (m << 8 | row tile, heaviest m first); K runs over latitude pairs.
+template <int D>
+__global__ __launch_bounds__(256) void leg_fwd_f64_wave_kernel(
+    const double *__restrict__ four, double *__restrict__ spec, const double *__restrict__ Pfwd,
+    const int *__restrict__ poff, const int *__restrict__ units, int N, int P2, int P2p, int ncol,
+    int nct, int nunits, FourGeom geo) {
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int u = blockIdx.x * 4 + wave;
+  if (u >= nunits) return;
+  const int ct = u % nct;
+  const int mk = units[u / nct];
+  const int m = mk >> 8, kt = mk & 255;
+  const int M1 = N + 1;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KR = (ke + 15) & ~15;
+  const int nsteps = P2p >> 2;
+  const double *__restrict__ W = Pfwd + poff[m] + kt * 16 + r16;
+  const int col = ct * 16 + r16;
+  const bool cok = col < ncol;
+  const int csafe = cok ? col : 0;
+
+  double a[2][D][2], b[2][D][2];
+  auto load = [&](int buf, int s0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int st = s0 + j, p = st * 4 + kq;
+      const int pa = st < nsteps ? p : kq;
+      const bool vb = cok && p < P2;
+      const int pb = vb ? p : 0;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        a[buf][j][q] = W[(size_t)(q * P2p + pa) * KR];
+        const double x = four[geo.at(pb, m, q, csafe)];
+        b[buf][j][q] = vb ? x : 0.0;
+      }
+    }
+  };
+  d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  auto run = [&](int buf, int s0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (s0 + j < nsteps) {
+        acc0 = mfma_f64(a[buf][j][0], b[buf][j][0], acc0);
+        acc1 = mfma_f64(a[buf][j][1], b[buf][j][1], acc1);
+      }
+  };
+  load(0, 0);
+  for (int s0 = 0; s0 < nsteps; s0 += 2 * D) {
+    load(1, s0 + D);
+    run(0, s0);
+    if (s0 + D >= nsteps) break;
+    load(0, s0 + 2 * D);
+    run(1, s0 + D);
+  }
+  if (cok) {
+    double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + col;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = kt * 16 + i * 4 + kq;
+      if (k < ke) sm[(size_t)(2 * k) * ncol] = acc0[i];
+      if (k < ko) sm[(size_t)(2 * k + 1) * ncol] = acc1[i];
+    }
+  }
+}
+
+// Tiled shape with a per-m K split.  A wave holds RT row tiles (RT*16 latitude
+// pairs) of one 16-column tile in registers, so each spectral fragment it
+// loads feeds RT MFMAs per parity.  The four waves of a workgroup split the K
+// loop of one column tile ks ways and cover 4/ks column tiles, with ks chosen
+// per m from the K length: 4 for the heavy low wavenumbers (short dependency
+// chains), 1 for the light high ones (no reduction, more columns per group).
+// Workgroups are numbered m-major, heaviest first: [0, m4) use ks = 4,
+// [m4, m2) ks = 2, [m2, N] ks = 1 (SHT_SHAPE_TILE: m4 = m2 = 0, no split;
+// SHT_SHAPE_TILE_SPLIT: thresholds on the K length).  Partial sums of a split
+// meet in LDS in FP64.
+struct KSplit {
+  int m4, m2;
+};
+
+template <int RT, int D>
+__global__ __launch_bounds__(256) void leg_inv_f64_tile_kernel(
+    const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
+    const double *__restrict__ Pinv, const double *__restrict__ Hinv,
+    const int *__restrict__ poff, const double *__restrict__ zero, int N, int P2, int P2p,
+    int ncol, int nct, int tab0, int ntab, KSplit kspl, const int *__restrict__ mlist, int mlen,
+    FourGeom geo) {
+  extern __shared__ double red[];  // [4 waves][RT][4][64], one parity at a time
+  const int M1 = N + 1;
+  const int ngr = (P2p >> 4) / RT;  // row groups
+  const int w4 = ngr * ntab * nct, w2 = ngr * ntab * ((nct + 1) >> 1), w1 = ngr * ntab * ((nct + 3) >> 2);
+  int b = blockIdx.x;
+  int ks, m, cpw;
+  if (kspl.m2 == 0) {  // no split: whole m per XCD from the balanced lists
+    ks = 1; cpw = 4;
+    if (!xcd_m_block(b, mlist, mlen, w1, m, b)) return;
+  } else if (b < kspl.m4 * w4) {  // split regions: round robin over the XCDs
+    ks = 4; cpw = 1; m = b / w4; b %= w4;
+  } else if ((b -= kspl.m4 * w4) < (kspl.m2 - kspl.m4) * w2) {
+    ks = 2; cpw = 2; m = kspl.m4 + b / w2; b %= w2;
+  } else {
+    b -= (kspl.m2 - kspl.m4) * w2;
+    ks = 1; cpw = 4; m = kspl.m2 + b / w1; b %= w1;
+  }
+  const int ncg = (nct + cpw - 1) / cpw;
+  const int ctg = b % ncg;
+  b /= ncg;
+  const int tab = tab0 + b % ntab;
+  const int rg = b / ntab;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int kw = wave % ks, cw = wave / ks;
+  const int ct = ctg * cpw + cw;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP = (ke + 3) & ~3, nsteps = KP >> 2;
+  const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m] + rg * RT * 16 + r16;
+  double *__restrict__ out = tab ? four2 : four;
+  const int se = tab ? 1 : 0, so = 1 - se;
+  const int col = ct * 16 + r16;
+  const bool cok = ct < nct && col < ncol;
+  const double *__restrict__ sp = spec + (size_t)coef_off(N, m) * ncol + (cok ? col : 0);
+
+  d4 acc[2][RT];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int r = 0; r < RT; ++r) acc[q][r] = d4{0.0, 0.0, 0.0, 0.0};
+  // This wave's K steps are kw, kw + ks, ... < nsteps, taken in chunks of D
+  // whose loads are all issued before the chunk's first MFMA.  Steps past the
+  // end load clamped addresses with a zero B operand, so every MFMA is
+  // unconditional (adding exact zeros) and no load can sink below a branch.
+  double a[1][D][2][RT], bv[1][D][2];
+  auto load = [&](int buf, int j0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int st = kw + ks * (j0 + j), k = st * 4 + kq;
+      const int kc = st < nsteps ? k : kq;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        const bool vb = cok && k < (q ? ko : ke);
+        bv[buf][j][q] = *(vb ? sp + (size_t)(2 * k + q) * ncol : zero);
+#pragma unroll
+        for (int r = 0; r < RT; ++r) a[buf][j][q][r] = T[(size_t)(q * KP + kc) * P2p + r * 16];
+      }
+    }
+  };
+  auto run = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int r = 0; r < RT; ++r) acc[q][r] = mfma_f64(a[buf][j][q][r], bv[buf][j][q], acc[q][r]);
+  };
+  for (int j0 = 0; kw + ks * j0 < nsteps; j0 += D) {
+    load(0, j0);
+    run(0);
+  }
+  if (ks == 1) {
+    if (cok) {
+#pragma unroll
+      for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int p = (rg * RT + r) * 16 + i * 4 + kq;
+          if (p < P2) {
+            __builtin_nontemporal_store(acc[0][r][i], out + geo.at(p, m, se, col));
+            __builtin_nontemporal_store(acc[1][r][i], out + geo.at(p, m, so, col));
+          }
+        }
+    }
+    return;
+  }
+  // ks = 2 or 4 (uniform over the workgroup): FP64 sums of the ks partials
+  const int per = RT * 256;  // doubles one wave holds per parity
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) red[wave * per + (r * 4 + i) * 64 + lane] = acc[q][r][i];
+    __syncthreads();
+    const int slot = q ? so : se;
+    for (int e = threadIdx.x; e < cpw * per; e += 256) {
+      const int g = e / per, el = e % per;
+      double v = red[(g * ks) * per + el];
+      for (int t = 1; t < ks; ++t) v += red[(g * ks + t) * per + el];
+      const int ln = el & 63, ri = el >> 6, i = ri & 3, r = ri >> 2;
+      const int p = (rg * RT + r) * 16 + i * 4 + (ln >> 4);
+      const int c = (ctg * cpw + g) * 16 + (ln & 15);
+      if (p < P2 && c < ncol) __builtin_nontemporal_store(v, out + geo.at(p, m, slot, c));
+    }
+    __syncthreads();
+  }
+}
+
+// Splits from the per-m K length: ks = 4 where nsteps(m) >= t4, 2 where >= t2.
+inline KSplit ksplit_inv(const Plan &p, int t4, int t2) {
+  KSplit k{0, 0};
+  for (int m = 0; m <= p.N; ++m) {
+    const int nsteps = roundup(ceil_half(p.N - m + 1), 4) / 4;
+    if (nsteps >= t4) k.m4 = m + 1;
+    if (nsteps >= t2) k.m2 = m + 1;
+  }
+  if (k.m2 < k.m4) k.m2 = k.m4;
+  return k;
+}
+
+template <int RT, int D>
+int inv_tile_launch(const Plan &p, KSplit ks, const double *spec, double *four, double *four2,
+                    int tab0, int ntab, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const int ngr = (p.P2p / 16) / RT;
+  const long w4 = (long)ngr * ntab * nct, w2 = (long)ngr * ntab * ((nct + 1) / 2),
+             w1 = (long)ngr * ntab * ((nct + 3) / 4);
+  const long nblk = ks.m2 == 0 ? xcd_m_blocks(p, 0, (int)w1)
+                               : ks.m4 * w4 + (long)(ks.m2 - ks.m4) * w2 + (long)(p.N + 1 - ks.m2) * w1;
+  const size_t lds = ks.m2 > 0 ? sizeof(double) * 4 * RT * 256 : 0;
+  leg_inv_f64_tile_kernel<RT, D><<<(int)nblk, 256, lds, s>>>(
+      spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.zero, p.N, p.P2, p.P2p, ncol, nct, tab0,
+      ntab, ks, p.xcd_m[0], p.xcd_mlen[0], four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+template <int D>
+int inv_wave_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0,
+                    int ntab, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const int upm = ntab * (p.P2p / 16) * nct;
+  leg_inv_f64_wave_kernel<D><<<xcd_m_blocks(p, 0, (upm + 3) / 4), 256, 0, s>>>(
+      spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p,
+      ncol, nct, tab0, ntab, four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+// Forward counterpart: a wave holds all 16-degree row tiles of its m (at most
+// MAXRT) for one column tile, so each Fourier fragment feeds every row tile;
+// the latitude-pair loop is split ks ways per m as in the inverse, with ks
+// chosen from the number of row tiles (the work of an m).
+template <int MAXRT, int D>
+__global__ __launch_bounds__(256) void leg_fwd_f64_tile_kernel(
+    const double *__restrict__ four, double *__restrict__ spec, const double *__restrict__ Pfwd,
+    const int *__restrict__ poff, const double *__restrict__ zero, int N, int P2, int P2p,
+    int ncol, int nct, KSplit kspl, const int *__restrict__ mlist, int mlen, FourGeom geo) {
+  extern __shared__ double red[];  // [4 waves][MAXRT][4][64], one parity at a time
+  const int M1 = N + 1;
+  const int w4 = nct, w2 = (nct + 1) >> 1, w1 = (nct + 3) >> 2;
+  int b = blockIdx.x;
+  int ks, m, cpw;
+  if (kspl.m2 == 0) {  // no split: whole m per XCD from the balanced lists
+    ks = 1; cpw = 4;
+    if (!xcd_m_block(b, mlist, mlen, w1, m, b)) return;
+  } else if (b < kspl.m4 * w4) {  // split regions: round robin over the XCDs
+    ks = 4; cpw = 1; m = b / w4; b %= w4;
+  } else if ((b -= kspl.m4 * w4) < (kspl.m2 - kspl.m4) * w2) {
+    ks = 2; cpw = 2; m = kspl.m4 + b / w2; b %= w2;
+  } else {
+    b -= (kspl.m2 - kspl.m4) * w2;
+    ks = 1; cpw = 4; m = kspl.m2 + b / w1; b %= w1;
+  }
+  const int ctg = b;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int kw = wave % ks, cw = wave / ks;
+  const int ct = ctg * cpw + cw;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KR = (ke + 15) & ~15, nrt = KR >> 4;
+  const int nsteps = P2p >> 2;
+  const double *__restrict__ W = Pfwd + poff[m] + r16;
+  const int col = ct * 16 + r16;
+  const bool cok = ct < nct && col < ncol;
+  const int csafe = cok ? col : 0;
+
+  d4 acc[2][MAXRT];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int r = 0; r < MAXRT; ++r) acc[q][r] = d4{0.0, 0.0, 0.0, 0.0};
+  // Chunks of D latitude-pair steps as in the inverse.  Row tiles past this
+  // m's last (r >= nrt) reuse tile 0's fragment and their results are never
+  // stored: unconditional MFMAs keep every load early.
+  double a[1][D][2][MAXRT], bv[1][D][2];
+  auto load = [&](int buf, int j0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const int st = kw + ks * (j0 + j), p = st * 4 + kq;
+      const int pa = st < nsteps ? p : kq;
+      const bool vb = cok && p < P2;
+      const int pb = vb ? p : 0;
+#pragma unroll
+      for (int q = 0; q < 2; ++q) {
+        bv[buf][j][q] = *(vb ? four + geo.at(pb, m, q, csafe) : zero);
+#pragma unroll
+        for (int r = 0; r < MAXRT; ++r)
+          a[buf][j][q][r] = W[(size_t)(q * P2p + pa) * KR + (r < nrt ? r : 0) * 16];
+      }
+    }
+  };
+  auto run = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int r = 0; r < MAXRT; ++r) acc[q][r] = mfma_f64(a[buf][j][q][r], bv[buf][j][q], acc[q][r]);
+  };
+  for (int j0 = 0; kw + ks * j0 < nsteps; j0 += D) {
+    load(0, j0);
+    run(0);
+  }
+  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol;
+  const int kcnt[2] = {ke, ko};
+  if (ks == 1) {
+    if (cok) {
+#pragma unroll
+      for (int r = 0; r < MAXRT; ++r)
+        if (r < nrt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int k = r * 16 + i * 4 + kq;
+            if (k < ke) __builtin_nontemporal_store(acc[0][r][i], sm + (size_t)(2 * k) * ncol + col);
+            if (k < ko) __builtin_nontemporal_store(acc[1][r][i], sm + (size_t)(2 * k + 1) * ncol + col);
+          }
+    }
+    return;
+  }
+  const int per = nrt * 256;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+#pragma unroll
+    for (int r = 0; r < MAXRT; ++r)
+      if (r < nrt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) red[wave * per + (r * 4 + i) * 64 + lane] = acc[q][r][i];
+    __syncthreads();
+    for (int e = threadIdx.x; e < cpw * per; e += 256) {
+      const int g = e / per, el = e % per;
+      double v = red[(g * ks) * per + el];
+      for (int t = 1; t < ks; ++t) v += red[(g * ks + t) * per + el];
+      const int ln = el & 63, ri = el >> 6, i = ri & 3, r = ri >> 2;
+      const int k = r * 16 + i * 4 + (ln >> 4);
+      const int c = (ctg * cpw + g) * 16 + (ln & 15);
+      if (k < kcnt[q] && c < ncol) __builtin_nontemporal_store(v, sm + (size_t)(2 * k + q) * ncol + c);
+    }
+    __syncthreads();
+  }
+}
+
+// ks = 4 where the m has at least t4 row tiles, 2 where at least t2.
+inline KSplit ksplit_fwd(const Plan &p, int t4, int t2) {
+  KSplit k{0, 0};
+  for (int m = 0; m <= p.N; ++m) {
+    const int nrt = roundup(ceil_half(p.N - m + 1), 16) / 16;
+    if (nrt >= t4) k.m4 = m + 1;
+    if (nrt >= t2) k.m2 = m + 1;
+  }
+  if (k.m2 < k.m4) k.m2 = k.m4;
+  return k;
+}
+
+template <int MAXRT, int D>
+int fwd_tile_launch(const Plan &p, KSplit ks, const double *four, double *spec, int nf,
+                    hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const long nblk = ks.m2 == 0 ? xcd_m_blocks(p, 1, (nct + 3) / 4)
+                               : (long)ks.m4 * nct + (long)(ks.m2 - ks.m4) * ((nct + 1) / 2) +
+                                     (long)(p.N + 1 - ks.m2) * ((nct + 3) / 4);
+  const size_t lds = ks.m2 > 0 ? sizeof(double) * 4 * MAXRT * 256 : 0;
+  leg_fwd_f64_tile_kernel<MAXRT, D><<<(int)nblk, 256, lds, s>>>(
+      four, spec, p.Pfwd, p.poff_fwd, p.zero, p.N, p.P2, p.P2p, ncol, nct, ks, p.xcd_m[1],
+      p.xcd_mlen[1], four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+template <int D>
+int fwd_wave_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const int nunits = p.n_fwd_units * nct;
+  leg_fwd_f64_wave_kernel<D><<<(nunits + 3) / 4, 256, 0, s>>>(
+      four, spec, p.Pfwd, p.poff_fwd, p.fwd_units, p.N, p.P2, p.P2p, ncol, nct, nunits,
+      four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 template <int RT>
 int inv_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0, int ntab,
                int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  const int nblk = (p.N + 1) * ntab * nct;
+  const int nblk = xcd_m_blocks(p, 0, ntab * nct);
   const size_t lds = sizeof(double) * kWaves * RT * 256;
-  leg_inv_f64_kernel<RT><<<nblk, 256, lds, s>>>(spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.N,
+  leg_inv_f64_kernel<RT><<<nblk, 256, lds, s>>>(spec, four, four2, p.Pinv, p.Hinv, p.poff_inv,
+                                                p.xcd_m[0], p.xcd_mlen[0], p.N,
                                                 p.P2, p.P2p, ncol, nct, tab0, ntab,
                                                 four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
@@ -203,9 +675,10 @@ int inv_launch(const Plan &p, const double *spec, double *four, double *four2, i
 template <int MAXRT>
 int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  const int nblk = (p.N + 1) * nct;
+  const int nblk = xcd_m_blocks(p, 1, nct);
   const size_t lds = sizeof(double) * kWaves * MAXRT * 256;
-  leg_fwd_f64_kernel<MAXRT><<<nblk, 256, lds, s>>>(four, spec, p.Pfwd, p.poff_fwd, p.N, p.P2,
+  leg_fwd_f64_kernel<MAXRT><<<nblk, 256, lds, s>>>(four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1],
+                                                   p.xcd_mlen[1], p.N, p.P2,
                                                    p.P2p, ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
@@ -213,11 +686,37 @@ int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStrea
 
 }  // namespace
 
-int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
-                       bool value_table, bool deriv_table, int nf, hipStream_t s) {
+int resolve_shape(const Plan &p, int nf, int dir) {
+  for (const Plan::ShapeEntry &e : p.shapes)
+    if (e.nf == nf) return dir ? e.fwd : e.inv;
+  const int d = dir ? p.shape_fwd : p.shape_inv;
+  if (d != SHT_SHAPE_AUTO) return d;
+  if (p.prec != SHT_PREC_FP64) return SHT_SHAPE_KSPLIT;
+  // measured at T85 on MI355X: register-blocked tiles win the inverse once
+  // there are enough column tiles to fill the chip; the K split wins the forward
+  if (dir == 1) return SHT_SHAPE_KSPLIT;
+  return nf >= 32 ? SHT_SHAPE_TILE : SHT_SHAPE_WAVE;
+}
+
+int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, double *four,
+                             double *four2, bool value_table, bool deriv_table, int nf,
+                             hipStream_t s) {
   if (!value_table && !deriv_table) return SHT_OK;
   const int tab0 = value_table ? 0 : 1;
   const int ntab = (value_table && deriv_table) ? 2 : 1;
+  if (shape == SHT_SHAPE_WAVE) return inv_wave_launch<4>(p, spec, four, four2, tab0, ntab, nf, s);
+  if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
+    const int rtiles = p.P2p / 16;
+    int rt = 4;
+    while (rtiles % rt) --rt;
+    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_inv(p, 6, 3);
+    switch (rt) {
+      case 4: return inv_tile_launch<4, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
+      case 3: return inv_tile_launch<3, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
+      case 2: return inv_tile_launch<2, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
+      default: return inv_tile_launch<1, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
+    }
+  }
   const int rtiles = p.P2p / 16;
   int rt = 8;
   while (rtiles % rt) --rt;
@@ -233,8 +732,24 @@ int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *
   }
 }
 
-int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
+int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
+                             hipStream_t s) {
+  if (shape == SHT_SHAPE_WAVE) return fwd_wave_launch<4>(p, four, spec, nf, s);
   const int rt = (ceil_half(p.N + 1) + 15) / 16;
+  if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
+    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_fwd(p, 2, 1);
+    switch (rt) {
+      case 1: return fwd_tile_launch<1, 2>(p, k, four, spec, nf, s);
+      case 2: return fwd_tile_launch<2, 2>(p, k, four, spec, nf, s);
+      case 3: return fwd_tile_launch<3, 2>(p, k, four, spec, nf, s);
+      case 4: return fwd_tile_launch<4, 2>(p, k, four, spec, nf, s);
+      case 5: return fwd_tile_launch<5, 2>(p, k, four, spec, nf, s);
+      case 6: return fwd_tile_launch<6, 2>(p, k, four, spec, nf, s);
+      case 7: return fwd_tile_launch<7, 2>(p, k, four, spec, nf, s);
+      case 8: return fwd_tile_launch<8, 2>(p, k, four, spec, nf, s);
+    }
+    return fail(SHT_ERR_UNSUPPORTED, "nmax above 255 is not instantiated");
+  }
   switch (rt) {
     case 1: return fwd_launch<1>(p, four, spec, nf, s);
     case 2: return fwd_launch<2>(p, four, spec, nf, s);
@@ -246,6 +761,16 @@ int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, 
     case 8: return fwd_launch<8>(p, four, spec, nf, s);
   }
   return fail(SHT_ERR_UNSUPPORTED, "nmax above 255 is not instantiated");
+}
+
+int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
+                       bool value_table, bool deriv_table, int nf, hipStream_t s) {
+  return launch_leg_inv_f64_shape(p, resolve_shape(p, nf, 0), spec, four, four2, value_table,
+                                  deriv_table, nf, s);
+}
+
+int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
+  return launch_leg_fwd_f64_shape(p, resolve_shape(p, nf, 1), four, spec, nf, s);
 }
 
 }  // namespace sht

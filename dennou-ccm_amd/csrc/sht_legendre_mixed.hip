@@ -127,32 +127,43 @@ __device__ __forceinline__ void slice_mac(const uint4 (&pa)[3], const uint4 (&pb
 // K values split into pieces, laid out [group][piece][col][32] (16-bit) so a
 // lane's 8 K values are one 16-byte LDS read; ebits[group][col] holds the FP16
 // scaling exponent.  group = par * maxsl + slice.
-// Thread t converts column t % 16 of group t / 16: it reads its 32 values
-// (load(k) returns the FP64 value or 0 outside the tile), takes their maximum
-// for the exponent, splits and stores them.
+// Every wave converts whole groups (wave w takes groups w, w+4, ...): lane l
+// reads the 8 values K = 8 (l/16) .. +7 of column l % 16 (load(g, k, c) returns
+// the FP64 value or 0 outside the tile), the FP16 exponent comes from the
+// maximum over the group's 32 values (a 4-lane shuffle reduction), and the
+// lane writes each of its pieces as one 16-byte LDS store -- already the
+// matrix-core B layout.
 template <int MODE, class Load>
 __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Load load) {
   constexpr int NP = Form<MODE>::NP;
-  const int t = threadIdx.x, c = t & 15, g = t >> 4;
-  if (g >= ngroups) return;
-  double v[32];
-  double mx = 0.0;
+  const int lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
+  for (int g = wave_id(); g < ngroups; g += 4) {
+    double v[8];
 #pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    v[k] = load(g, k, c);
-    mx = fmax(mx, fabs(v[k]));
-  }
-  const int e = (Form<MODE>::kScaled && mx > 0.0) ? 13 - ilogb(mx) : 0;
-  ebits[g * 16 + c] = e;
-  uint16_t *dst = bt + ((size_t)g * NP * 16 + c) * 32;
+    for (int i = 0; i < 8; ++i) v[i] = load(g, k8 * 8 + i, c);
+    int e = 0;
+    if constexpr (Form<MODE>::kScaled) {
+      double mx = 0.0;
 #pragma unroll
-  for (int k = 0; k < 32; k += 2) {
-    uint16_t p0[3], p1[3];
-    split<MODE>(Form<MODE>::kScaled ? ldexp(v[k], e) : v[k], p0);
-    split<MODE>(Form<MODE>::kScaled ? ldexp(v[k + 1], e) : v[k + 1], p1);
+      for (int i = 0; i < 8; ++i) mx = fmax(mx, fabs(v[i]));
+      mx = fmax(mx, __shfl_xor(mx, 16));
+      mx = fmax(mx, __shfl_xor(mx, 32));
+      e = mx > 0.0 ? 13 - ilogb(mx) : 0;
+      if (k8 == 0) ebits[g * 16 + c] = e;
+    }
+    uint32_t w[3][4];
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      uint16_t p0[3], p1[3];
+      split<MODE>(Form<MODE>::kScaled ? ldexp(v[i], e) : v[i], p0);
+      split<MODE>(Form<MODE>::kScaled ? ldexp(v[i + 1], e) : v[i + 1], p1);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) w[q][i >> 1] = p0[q] | ((uint32_t)p1[q] << 16);
+    }
 #pragma unroll
     for (int q = 0; q < NP; ++q)
-      *reinterpret_cast<uint32_t *>(dst + (size_t)q * 16 * 32 + k) = p0[q] | ((uint32_t)p1[q] << 16);
+      *reinterpret_cast<uint4 *>(bt + (((size_t)g * NP + q) * 16 + c) * 32 + k8 * 8) =
+          make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
   }
 }
 
@@ -176,15 +187,15 @@ __device__ __forceinline__ void load_pb(const uint16_t *bt, int g, int kq, int r
 template <int MODE, int NSL>
 __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
-    const int *__restrict__ loff, const int *__restrict__ lexp, int N, int P2, int P2p, int ncol,
-    int nct, FourGeom geo) {
+    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
+    int mlen, int N, int P2, int P2p, int ncol, int nct, FourGeom geo) {
   constexpr int NP = Form<MODE>::NP;
   __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * 32];
   __shared__ int ebits[2 * NSL * 16];
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int ct = b % nct, m = b / nct;
+  int m, ct;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
   const int M1 = N + 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
   const int col0 = ct * 16;
@@ -253,15 +264,15 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
 template <int MODE, int NSL>
 __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
     const float *__restrict__ four, double *__restrict__ spec, const uint16_t *__restrict__ L,
-    const int *__restrict__ loff, const int *__restrict__ lexp, int N, int P2, int P2p32, int ncol,
-    int nct, FourGeom geo) {
+    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
+    int mlen, int N, int P2, int P2p32, int ncol, int nct, FourGeom geo) {
   constexpr int NP = Form<MODE>::NP;
   __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * 32];
   __shared__ int ebits[2 * NSL * 16];
-  const int b = xcd_tile(blockIdx.x, gridDim.x);
-  const int ct = b % nct, m = b / nct;
+  int m, ct;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
   const int M1 = N + 1;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
   const int nsl = P2p32 >> 5;
@@ -410,8 +421,9 @@ int build_mixed_mode(Plan &p) {
 template <int MODE, int NSL>
 int inv_nsl(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  leg_inv_mixed_kernel<MODE, NSL><<<(p.N + 1) * nct, 256, 0, s>>>(
-      spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.N, p.P2, p.P2p,
+  leg_inv_mixed_kernel<MODE, NSL><<<xcd_m_blocks(p, 0, nct), 256, 0, s>>>(
+      spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
+      p.xcd_mlen[0], p.N, p.P2, p.P2p,
       ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
@@ -431,8 +443,9 @@ int inv_mode(const Plan &p, const double *spec, float *four, int nf, hipStream_t
 template <int MODE, int NSL>
 int fwd_nsl(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  leg_fwd_mixed_kernel<MODE, NSL><<<(p.N + 1) * nct, 256, 0, s>>>(
-      four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.N, p.P2,
+  leg_fwd_mixed_kernel<MODE, NSL><<<xcd_m_blocks(p, 1, nct), 256, 0, s>>>(
+      four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
+      p.xcd_mlen[1], p.N, p.P2,
       roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;

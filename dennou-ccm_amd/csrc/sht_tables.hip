@@ -3,6 +3,8 @@
 // table, and FFT twiddles.  All of it stays FP64 whatever the precision of the
 // Legendre GEMMs (SPML wa_Initial's work, run inside DynamicsHSplVAS83Init at
 // atm/dcpam_main_mod.f90:5036).
+#include <algorithm>
+
 #include "sht_internal.h"
 
 namespace sht {
@@ -158,6 +160,8 @@ int build_tables(Plan &p) {
   SHT_HIP(hipMalloc(&p.w, sizeof(double) * p.jm));
   SHT_HIP(hipMalloc(&p.coslat, sizeof(double) * p.jm));
   SHT_HIP(hipMalloc(&p.twid, sizeof(double) * 2 * p.im));
+  SHT_HIP(hipMalloc(&p.zero, sizeof(double) * 64));
+  SHT_HIP(hipMemset(p.zero, 0, sizeof(double) * 64));
   SHT_HIP(hipMalloc(&p.Pinv, sizeof(double) * ti));
   SHT_HIP(hipMalloc(&p.Pfwd, sizeof(double) * tf));
   if (p.flags & SHT_FLAG_GRAD) SHT_HIP(hipMalloc(&p.Hinv, sizeof(double) * ti));
@@ -168,6 +172,38 @@ int build_tables(Plan &p) {
   SHT_HIP(hipMemset(p.Pinv, 0, sizeof(double) * ti));
   SHT_HIP(hipMemset(p.Pfwd, 0, sizeof(double) * tf));
   if (p.Hinv) SHT_HIP(hipMemset(p.Hinv, 0, sizeof(double) * ti));
+  std::vector<int> units;
+  for (int m = 0; m <= N; ++m)
+    for (int kt = 0; kt < roundup(ceil_half(N - m + 1), 16) / 16; ++kt) units.push_back(m << 8 | kt);
+  p.n_fwd_units = (int)units.size();
+  SHT_HIP(hipMalloc(&p.fwd_units, sizeof(int) * units.size()));
+  SHT_HIP(hipMemcpy(p.fwd_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice));
+  for (int dir = 0; dir < 2; ++dir) {
+    std::vector<int> work(N + 1), order(N + 1);
+    for (int m = 0; m <= N; ++m) {
+      const int ke = ceil_half(N - m + 1);
+      work[m] = dir == 0 ? roundup(ke, 4) / 4 : roundup(ke, 16) / 16;
+      order[m] = m;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    std::vector<std::vector<int>> lists(8);
+    std::vector<long> load(8, 0);
+    for (int m : order) {
+      int x = 0;
+      for (int y = 1; y < 8; ++y)
+        if (load[y] < load[x]) x = y;
+      lists[x].push_back(m);
+      load[x] += work[m];
+    }
+    size_t len = 0;
+    for (auto &l : lists) len = std::max(len, l.size());
+    std::vector<int> flat(8 * len, -1);
+    for (int x = 0; x < 8; ++x)
+      for (size_t j = 0; j < lists[x].size(); ++j) flat[x * len + j] = lists[x][j];
+    p.xcd_mlen[dir] = (int)len;
+    SHT_HIP(hipMalloc(&p.xcd_m[dir], sizeof(int) * flat.size()));
+    SHT_HIP(hipMemcpy(p.xcd_m[dir], flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
+  }
 
   gauss_kernel<<<(P2 + 63) / 64, 64>>>(p.jm, p.mu, p.w, p.coslat);
   SHT_HIP(hipGetLastError());

@@ -26,6 +26,9 @@ SHT_ERR_BUSY = -5
 
 PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16, PREC_SPLIT_BF16X6 = 0, 1, 2, 3
 DIR_INVERSE, DIR_FORWARD, DIR_ROUNDTRIP = 0, 1, 2
+SHAPE_AUTO, SHAPE_KSPLIT, SHAPE_WAVE, SHAPE_TILE, SHAPE_TILE_SPLIT = -1, 0, 1, 2, 3
+SHAPES = ("ksplit", "wave", "tile", "tile_split")
+NSHAPE = len(SHAPES)
 FLAG_GRAD = 1
 
 _lib = None
@@ -94,6 +97,9 @@ SIGNATURES = {
     "sht_server_stats": (_ci, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "sht_plan_set_timing": (_ci, [_vp, _ci]),
     "sht_plan_stage_times": (_ci, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "sht_plan_set_shapes": (_ci, [_vp, _ci, _ci, _ci]),
+    "sht_plan_get_shapes": (_ci, [_vp, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
+    "sht_plan_autotune": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _dp]),
 }
 
 STAGES = ("leg_inv", "fft_inv", "fft_fwd", "leg_fwd")
@@ -206,6 +212,24 @@ class Plan:
         n = (ctypes.c_longlong * 4)()
         check(lib().sht_plan_stage_times(self._h, ms, n))
         return {STAGES[i]: (ms[i], n[i]) for i in range(4)}
+
+    # ---- Legendre kernel shapes
+    def set_shapes(self, nfield: int, inv: int, fwd: int):
+        """Fix the shapes for calls with nfield fields (0: the plan default)."""
+        check(lib().sht_plan_set_shapes(self._h, nfield, inv, fwd))
+
+    def get_shapes(self, nfield: int):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().sht_plan_get_shapes(self._h, nfield, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, nfield: int, reps: int = 20, cache_path: str | None = None):
+        """Time every shape at nfield fields and keep the fastest.  Returns
+        ((inv_shape, fwd_shape), (inv_us, fwd_us)); the times are 0 on a cache hit."""
+        us = (ctypes.c_double * 2)()
+        path = cache_path.encode() if cache_path else None
+        check(lib().sht_plan_autotune(self._h, nfield, reps, path, us))
+        return self.get_shapes(nfield), (us[0], us[1])
 
     # ---- host drop-in (SPML packing), numpy arrays
     def _host_call(self, fn, inp, out_shape):

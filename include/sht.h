@@ -214,6 +214,35 @@ int sht_plan_set_timing(sht_plan *plan, int enable);
  * launch counts (arrays of SHT_NSTAGE; either may be NULL), then resets. */
 int sht_plan_stage_times(sht_plan *plan, double *total_ms, long long *launches);
 
+/* ---- Legendre kernel shapes and their autotuner.  No SPML counterpart: the
+ *      best tiling of the per-m GEMMs depends on the truncation and on the
+ *      number of fields per call, which the ensemble and serving paths vary.
+ *      Shapes differ in the order of the K sums, so their results agree to
+ *      rounding; for one shape a result depends neither on nf nor on the
+ *      field's position in the batch.                                       */
+enum {
+  SHT_SHAPE_AUTO = -1,        /* the tuned or set entry for nf, else the built-in choice */
+  SHT_SHAPE_KSPLIT = 0,       /* workgroup per (m, 16 columns); its 4 waves split K     */
+  SHT_SHAPE_WAVE = 1,         /* wavefront per 16x16 tile, whole K, pipelined loads     */
+  SHT_SHAPE_TILE = 2,         /* wavefront per 16 columns x all row tiles (registers)   */
+  SHT_SHAPE_TILE_SPLIT = 3,   /* TILE with K split 4 or 2 ways for the heavy low m      */
+  SHT_NSHAPE = 4
+};
+/* Fixes the shapes for calls with nf fields (nf = 0: the plan default for every
+ * count without an entry).  FP64 plans take every shape; mixed-precision plans
+ * take SHT_SHAPE_AUTO and SHT_SHAPE_KSPLIT. */
+int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
+/* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
+int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);
+/* Times every shape of both Legendre kernels at nf fields on the plan's device
+ * (reps launches each, on a private stream; synchronizes it) and keeps the
+ * fastest for calls with nf fields.  With cache_path not NULL, a JSON file
+ * keyed by truncation, grid, nf and precision is consulted first and updated
+ * after timing.  best_us (2 doubles, may be NULL) receives the winners' mean
+ * durations in microseconds (0 when taken from the cache).  It uses the plan's
+ * scratch: call it while no other call on the plan is in flight. */
+int sht_plan_autotune(sht_plan *plan, int nf, int reps, const char *cache_path, double *best_us);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,168 @@
+"""Legendre kernel shapes and the autotuner, through the C ABI.
+
+Every shape of both FP64 Legendre kernels must meet the FP64 bar against the
+CPU oracle (relative L2 <= 1e-12) on full-size, ragged, tiny and unpadded
+problems, including the derivative table; the shapes differ only in the order
+of the K sums, so they agree with each other to a few ulps.  The autotuner
+must keep a valid shape, write its JSON cache and take the cache on a second
+plan without timing.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import shtamd
+from shtamd import synth
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+SHAPES = list(range(shtamd.NSHAPE))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def run_inverse(plan, spec_c, nf, grad=False):
+    s = dev(shtamd.spec_to_real(spec_c))
+    outs = [torch.empty((nf, plan.jm, plan.im), dtype=torch.float64, device="cuda") for _ in range(3)]
+    if grad:
+        plan.spec_to_grad(s, *outs, nf)
+    else:
+        plan.spec_to_grid(s, outs[0], nf)
+    torch.cuda.synchronize()
+    return [o.cpu().numpy() for o in (outs if grad else outs[:1])]
+
+
+def run_forward(plan, grid, nf):
+    s = torch.empty((plan.K, nf, 2), dtype=torch.float64, device="cuda")
+    plan.grid_to_spec(dev(grid), s, nf)
+    torch.cuda.synchronize()
+    return shtamd.real_to_spec(s.cpu().numpy())
+
+
+CASES = [  # (N, im, jm, nf): full configs, ragged tiles, tiny and unpadded grids
+    (85, 256, 128, 97),
+    (170, 512, 256, 13),
+    (42, 128, 64, 1),
+    (42, 128, 64, 9),
+    (21, 64, 32, 49),
+    (0, 16, 2, 3),
+    (7, 16, 10, 5),
+    (31, 64, 48, 17),
+]
+
+
+@pytest.mark.parametrize("N,im,jm,nf", CASES)
+def test_every_shape_matches_oracle(N, im, jm, nf):
+    spec = synth.spectra(N, nf, cfg=5)
+    spec[0] = np.linspace(1.0, 2.0, nf)
+    ref = oracle.Plan(N, im, jm)
+    g_ref = ref.spec_to_grid(spec)
+    s_ref = ref.grid_to_spec(g_ref)
+    p = shtamd.Plan(N, im, jm, nf)
+    got = {}
+    for sh in SHAPES:
+        say("shape", sh, "case", N, im, jm, nf)
+        p.set_shapes(nf, sh, sh)
+        assert p.get_shapes(nf) == (sh, sh)
+        g = run_inverse(p, spec, nf)[0]
+        s = run_forward(p, g_ref, nf)
+        assert rel(g, g_ref) < TOL, sh
+        assert rel(s, s_ref) < TOL, sh
+        got[sh] = (g, s)
+    for sh in SHAPES[1:]:
+        assert rel(got[sh][0], got[0][0]) < 1e-14
+        assert rel(got[sh][1], got[0][1]) < 1e-14
+    p.close()
+
+
+@pytest.mark.parametrize("N,im,jm,nlev", [(42, 128, 64, 26), (7, 16, 10, 2)])
+def test_every_shape_gradients(N, im, jm, nlev):
+    nf = 3 * nlev + 1
+    spec = synth.state_spectra(N, nlev, cfg=1)
+    ref = oracle.Plan(N, im, jm)
+    p = shtamd.Plan(N, im, jm, nf, flags=shtamd.FLAG_GRAD)
+    for sh in SHAPES:
+        say("grad shape", sh, N, nf)
+        p.set_shapes(0, sh, sh)
+        outs = run_inverse(p, spec, nf, grad=True)
+        for kind, o in enumerate(outs):
+            assert rel(o, ref.spec_to_grid(spec, kind=kind)) < TOL, (sh, kind)
+    p.close()
+
+
+def test_shape_results_do_not_depend_on_batch():
+    """For one shape, a field's result is bitwise the same alone and in a batch."""
+    N, im, jm, nf = 42, 128, 64, 40
+    spec = synth.spectra(N, nf, cfg=6)
+    p = shtamd.Plan(N, im, jm, nf)
+    for sh in SHAPES:
+        say("batch independence", sh)
+        p.set_shapes(0, sh, sh)
+        full = run_inverse(p, spec, nf)[0]
+        one = run_inverse(p, spec[:, 17:18], 1)[0]
+        assert np.array_equal(full[17:18], one)
+        sf = run_forward(p, full, nf)
+        s1 = run_forward(p, full[17:18], 1)
+        assert np.array_equal(sf[:, 17:18], s1)
+    p.close()
+
+
+def test_autotune_and_cache(tmp_path):
+    N, im, jm, nf = 85, 256, 128, 97
+    cache = str(tmp_path / "shapes.json")
+    p = shtamd.Plan(N, im, jm, nf)
+    say("autotune", N, nf)
+    (inv, fwd), (ti, tf) = p.autotune(nf, reps=10, cache_path=cache)
+    assert inv in SHAPES and fwd in SHAPES
+    assert ti > 0 and tf > 0
+    data = json.load(open(cache))
+    assert data[f"T{N}_{im}x{jm}_nf{nf}_p0"] == [inv, fwd]
+    # other field counts keep the default; an entry for nf only
+    assert p.get_shapes(nf) == (inv, fwd)
+    # a second plan takes the cached choice without timing
+    q = shtamd.Plan(N, im, jm, nf)
+    (inv2, fwd2), (t2i, t2f) = q.autotune(nf, reps=10, cache_path=cache)
+    assert (inv2, fwd2) == (inv, fwd) and t2i == 0.0 and t2f == 0.0
+    # and the tuned plan still meets the FP64 bar
+    spec = synth.spectra(N, nf, cfg=7)
+    ref = oracle.Plan(N, im, jm)
+    assert rel(run_inverse(q, spec, nf)[0], ref.spec_to_grid(spec)) < TOL
+    p.close()
+    q.close()
+
+
+def test_shape_argument_checks():
+    p = shtamd.Plan(21, 64, 32, 8)
+    with pytest.raises(shtamd.ShtError):
+        p.set_shapes(0, 99, 0)
+    with pytest.raises(shtamd.ShtError):
+        p.set_shapes(9, 0, 0)  # nf above kmax
+    p.set_shapes(4, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_KSPLIT)
+    assert p.get_shapes(4) == (shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_KSPLIT)
+    p.set_shapes(4, shtamd.SHAPE_AUTO, shtamd.SHAPE_AUTO)
+    assert p.get_shapes(4) == p.get_shapes(0)
+    p.close()
+    m = shtamd.Plan(21, 64, 32, 8, precision=shtamd.PREC_SPLIT_BF16X6)
+    with pytest.raises(shtamd.ShtError):
+        m.set_shapes(0, shtamd.SHAPE_WAVE, shtamd.SHAPE_WAVE)
+    m.set_shapes(0, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT)
+    (inv, fwd), _ = m.autotune(8, reps=2)
+    assert (inv, fwd) == (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT)
+    m.close()
