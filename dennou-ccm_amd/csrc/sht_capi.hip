@@ -46,7 +46,8 @@ static void release(Plan &p) {
   for (hipEvent_t e : p.event_pool) (void)hipEventDestroy(e);
   void *ptrs[] = {p.mu, p.w, p.coslat, p.twid, p.Pinv, p.Hinv, p.Pfwd, p.poff_inv, p.poff_fwd,
                   p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.lexp_inv, p.lexp_fwd, p.four, p.four2,
-                  p.stage_a, p.stage_b, p.fwd_units, p.zero, p.xcd_m[0], p.xcd_m[1]};
+                  p.stage_a, p.stage_b, p.fwd_units, p.zero, p.xcd_m[0], p.xcd_m[1],
+                  p.Hfwd};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p.pinned) (void)hipHostFree(p.pinned);
@@ -87,16 +88,27 @@ static hipEvent_t pooled_event(Plan &p) {
   return e;
 }
 
-// Runs one launcher; with timing on, brackets it with events on its stream.
+thread_local StageTimer *tl_stage_timer = nullptr;
+
+// Runs one launcher; with timing on, its kernels record the stage's events
+// themselves (launch_kernel), from the first kernel's start to the last's end.
 template <class F>
 static int timed(Plan &p, int stage, hipStream_t s, F &&launch) {
   if (!p.timing) return launch();
-  hipEvent_t a = pooled_event(p), b = pooled_event(p);
-  if (!a || !b) return fail(SHT_ERR_DEVICE, "stage timer: hipEventCreate failed");
-  SHT_HIP(hipEventRecord(a, s));
+  StageTimer t;
+  t.start = pooled_event(p);
+  t.stop = pooled_event(p);
+  if (!t.start || !t.stop) return fail(SHT_ERR_DEVICE, "stage timer: hipEventCreate failed");
+  tl_stage_timer = &t;
   int rc = launch();
-  SHT_HIP(hipEventRecord(b, s));
-  p.timed.push_back({stage, a, b});
+  tl_stage_timer = nullptr;
+  (void)s;
+  if (t.used) {
+    p.timed.push_back({stage, t.start, t.stop});
+  } else {
+    p.event_pool.push_back(t.start);
+    p.event_pool.push_back(t.stop);
+  }
   return rc;
 }
 
@@ -135,6 +147,25 @@ static int spec_to_grad(Plan &p, const double *spec, double *grid, double *glon,
   if (rc) return rc;
   return timed(p, SHT_STAGE_FFT_INV, s, [&] {
     return launch_fft_inv(p, p.four, p.four2, false, grid, glon, glat, nf, s);
+  });
+}
+
+// Flux divergence (1/cos) dA/dlon + (1/cos) d(B cos)/dlat to spectra, FP64 in
+// every precision form: A's Fourier coefficients times i m / cos meet the
+// value table, B's times 1 / cos meet -(w/2)(1-mu^2) dPt/dmu (the lat term
+// integrated by parts) with the parity slots swapped; the second GEMM adds
+// into the first's output.
+static int grid_to_div(Plan &p, const double *a, const double *b, double *spec, int nf,
+                       hipStream_t s) {
+  double *fa = a ? p.four : nullptr;
+  double *fb = b ? (a ? p.four2 : p.four) : nullptr;
+  int rc = timed(p, SHT_STAGE_FFT_FWD, s, [&] { return launch_fft_fwd_div(p, a, fa, b, fb, nf, s); });
+  if (rc) return rc;
+  return timed(p, SHT_STAGE_LEG_FWD, s, [&] {
+    int r = SHT_OK;
+    if (a) r = launch_leg_fwd_f64_table(p, p.Pfwd, fa, spec, false, false, nf, s);
+    if (!r && b) r = launch_leg_fwd_f64_table(p, p.Hfwd, fb, spec, true, a != nullptr, nf, s);
+    return r;
   });
 }
 
@@ -291,6 +322,23 @@ int sht_spec_to_grad_dev(sht_plan *h, const double *spec, double *grid, double *
   return spec_to_grad(h->p, spec, grid, gradlon, gradlat, nf, (hipStream_t)stream);
 }
 
+int sht_grid_to_div_dev(sht_plan *h, const double *a, const double *b, double *spec, int nf,
+                        void *stream) {
+  int rc = check_call(h, a ? a : b, spec, nf, "sht_grid_to_div_dev");
+  if (rc) return rc;
+  if (b && !(h->p.flags & SHT_FLAG_GRAD))
+    return fail(SHT_ERR_INVALID, "sht_grid_to_div_dev: the lat term needs a plan built with SHT_FLAG_GRAD");
+  return grid_to_div(h->p, a, b, spec, nf, (hipStream_t)stream);
+}
+
+int sht_spec_lapla_dev(const sht_plan *h, const double *in, double *out, int power, int nf,
+                       void *stream) {
+  int rc = check_call(h, in, out, nf, "sht_spec_lapla_dev");
+  if (rc) return rc;
+  if (power < -4 || power > 4) return fail(SHT_ERR_INVALID, "sht_spec_lapla_dev: power must be in [-4, 4]");
+  return launch_spec_lapla(h->p, in, out, power, nf, (hipStream_t)stream);
+}
+
 int sht_spml_to_canon_dev(const sht_plan *h, const double *wa, double *spec, int nf, void *stream) {
   int rc = check_call(h, wa, spec, nf, "sht_spml_to_canon_dev");
   if (rc) return rc;
@@ -310,22 +358,23 @@ int sht_avr_lonlat_dev(const sht_plan *h, const double *grid, double *avg, int n
 }
 
 // ---- host drop-in (SPML packing, synchronous)
-enum HostOp { kXyaWa, kWaXya, kGradLon, kGradLat };
+enum HostOp { kXyaWa, kWaXya, kGradLon, kGradLat, kDivLon, kDivLat };
 
 static int host_call(sht_plan *h, const double *in, double *out, int nlev, HostOp op,
                      const char *who) {
   int rc = check_call(h, in, out, nlev, who);
   if (rc) return rc;
   Plan &p = h->p;
-  if (op == kGradLat && !(p.flags & SHT_FLAG_GRAD))
+  if ((op == kGradLat || op == kDivLat) && !(p.flags & SHT_FLAG_GRAD))
     return fail(SHT_ERR_INVALID, std::string(who) + ": needs a plan built with SHT_FLAG_GRAD");
   rc = ensure_host_staging(p);
   if (rc) return rc;
   hipStream_t s = p.host_stream;
   const size_t ngrid = (size_t)nlev * p.jm * p.im;
   const size_t nwa = (size_t)nlev * (p.N + 1) * (p.N + 1);
-  const size_t nin = (op == kWaXya) ? ngrid : nwa;
-  const size_t nout = (op == kWaXya) ? nwa : ngrid;
+  const bool to_spec = op == kWaXya || op == kDivLon || op == kDivLat;
+  const size_t nin = to_spec ? ngrid : nwa;
+  const size_t nout = to_spec ? nwa : ngrid;
   std::memcpy(p.pinned, in, sizeof(double) * nin);
   SHT_HIP(hipMemcpyAsync(p.stage_a, p.pinned, sizeof(double) * nin, hipMemcpyHostToDevice, s));
   switch (op) {
@@ -345,6 +394,14 @@ static int host_call(sht_plan *h, const double *in, double *out, int nlev, HostO
       if ((rc = grid_to_spec(p, p.stage_a, p.stage_b, nlev, s))) return rc;
       if ((rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s))) return rc;
       break;
+    case kDivLon:
+    case kDivLat: {
+      const double *a = op == kDivLon ? p.stage_a : nullptr;
+      const double *b = op == kDivLat ? p.stage_a : nullptr;
+      if ((rc = grid_to_div(p, a, b, p.stage_b, nlev, s))) return rc;
+      if ((rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s))) return rc;
+      break;
+    }
   }
   SHT_HIP(hipMemcpyAsync(p.pinned, p.stage_a, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
   SHT_HIP(hipStreamSynchronize(s));
@@ -363,6 +420,34 @@ int sht_xya_gradlon_wa(sht_plan *h, const double *wa, double *xya, int nlev) {
 }
 int sht_xya_gradlat_wa(sht_plan *h, const double *wa, double *xya, int nlev) {
   return host_call(h, wa, xya, nlev, kGradLat, "sht_xya_gradlat_wa");
+}
+int sht_wa_divlon_xya(sht_plan *h, const double *xya, double *wa, int nlev) {
+  return host_call(h, xya, wa, nlev, kDivLon, "sht_wa_divlon_xya");
+}
+int sht_wa_divlat_xya(sht_plan *h, const double *xya, double *wa, int nlev) {
+  return host_call(h, xya, wa, nlev, kDivLat, "sht_wa_divlat_xya");
+}
+
+// Diagonal operators on the SPML packing, applied on the host (no transfer):
+// slot l = n(n+1)+m+1 scales by (-n(n+1))^power, n = 0 to zero for power < 0.
+static int host_lapla(sht_plan *h, const double *in, double *out, int nlev, int power,
+                      const char *who) {
+  int rc = check_call(h, in, out, nlev, who);
+  if (rc) return rc;
+  const int N = h->p.N, L = (N + 1) * (N + 1);
+  for (int n = 0; n <= N; ++n) {
+    const double e = -(double)n * (n + 1);
+    const double fac = power < 0 ? (n == 0 ? 0.0 : 1.0 / e) : e;
+    for (int f = 0; f < nlev; ++f)
+      for (int l = n * n; l < (n + 1) * (n + 1); ++l) out[(size_t)f * L + l] = fac * in[(size_t)f * L + l];
+  }
+  return SHT_OK;
+}
+int sht_wa_lapla_wa(sht_plan *h, const double *wa, double *out, int nlev) {
+  return host_lapla(h, wa, out, nlev, 1, "sht_wa_lapla_wa");
+}
+int sht_wa_laplainv_wa(sht_plan *h, const double *wa, double *out, int nlev) {
+  return host_lapla(h, wa, out, nlev, -1, "sht_wa_laplainv_wa");
 }
 
 }  // extern "C"

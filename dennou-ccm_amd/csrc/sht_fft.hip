@@ -169,14 +169,24 @@ __device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw) {
   }
 }
 
-// grid -> four (S, A halves).  Blocks: P2 * ceil(nf / G).
+template <typename TF>
+struct FwdIO {
+  const double *src[2];
+  TF *dst[2];
+  int mode[2];  // 0 value, 1 divergence lon term (x i m / cos), 2 lat term (x 1 / cos)
+};
+
+// grid -> four (S, A halves).  Blocks: P2 * ceil(nf / G) x inputs.
 // All global loads of a thread (its 2 * G*L/256 ring samples and its twiddles)
 // are issued before the first LDS store, so a block pays one HBM latency.
 template <int L, typename TF>
-__global__ __launch_bounds__(kThreads) void fft_fwd_kernel(const double *__restrict__ grid,
-                                                           TF *__restrict__ four,
-                                                           const double2 *__restrict__ tw, int P2,
-                                                           int M1, int nf, int jm) {
+__global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
+                                                           const double2 *__restrict__ tw,
+                                                           const double *__restrict__ coslat,
+                                                           int P2, int M1, int nf, int jm) {
+  const double *__restrict__ grid = io.src[blockIdx.y];
+  TF *__restrict__ four = io.dst[blockIdx.y];
+  const int mode = io.mode[blockIdx.y];
   constexpr int G = kLdsComplex / L;
   constexpr int NQ = kLdsComplex / kThreads; // samples per thread
   __shared__ double2 buf[kLdsPadded];
@@ -202,7 +212,7 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(const double *__restr
   for (int q = 0; q < NQ; ++q) buf[px(tid + q * kThreads)] = z[q];
   __syncthreads();
   fft_lds<L, -1>(buf, twl);
-  const double sc = 0.5 / L;
+  const double sc = 0.5 / L * (mode == 0 ? 1.0 : 1.0 / coslat[jn]);
   for (int idx = tid; idx < M1 * G; idx += kThreads) {
     const int m = idx / G, g = idx % G, f = f0 + g;
     if (f >= nf) continue;
@@ -212,6 +222,10 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(const double *__restr
     const double2 xn = make_double2((zm.x + zc.x) * sc, (zm.y - zc.y) * sc);
     const double2 xs = make_double2((zm.y + zc.y) * sc, -(zm.x - zc.x) * sc);
     double2 S = cadd(xn, xs), A = csub(xn, xs);
+    if (mode == 1) {  // x i m
+      S = make_double2(-m * S.y, m * S.x);
+      A = make_double2(-m * A.y, m * A.x);
+    }
     if (m == 0) S.y = A.y = 0.0;
     TF *o = four + ((((size_t)p * nfg + fgr) * M1 + m) * 2) * (2 * G) + 2 * g;
     store_pair(o, S);
@@ -301,11 +315,11 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
 }
 
 template <int L, typename TF>
-int fwd_dispatch(const Plan &p, const double *grid, TF *four, int nf, hipStream_t s) {
+int fwd_dispatch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  fft_fwd_kernel<L, TF><<<nblk, kThreads, 0, s>>>(grid, four, reinterpret_cast<const double2 *>(p.twid),
-                                                  p.P2, p.N + 1, nf, p.jm);
+  launch_kernel(fft_fwd_kernel<L, TF>, dim3(nblk, nin), kThreads, 0, s,
+      io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -314,23 +328,24 @@ template <int L, typename TF>
 int inv_dispatch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  fft_inv_kernel<L, TF><<<dim3(nblk, nout), kThreads, 0, s>>>(
+  launch_kernel(fft_inv_kernel<L, TF>, dim3(nblk, nout), kThreads, 0, s,
       io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
 
 template <typename TF>
-int fwd_typed(const Plan &p, const double *grid, TF *four, int nf, hipStream_t s) {
+int fwd_typed(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
+  if (nin == 0) return SHT_OK;
   switch (p.im) {
-    case 16: return fwd_dispatch<16>(p, grid, four, nf, s);
-    case 32: return fwd_dispatch<32>(p, grid, four, nf, s);
-    case 64: return fwd_dispatch<64>(p, grid, four, nf, s);
-    case 128: return fwd_dispatch<128>(p, grid, four, nf, s);
-    case 256: return fwd_dispatch<256>(p, grid, four, nf, s);
-    case 512: return fwd_dispatch<512>(p, grid, four, nf, s);
-    case 1024: return fwd_dispatch<1024>(p, grid, four, nf, s);
-    case 2048: return fwd_dispatch<2048>(p, grid, four, nf, s);
+    case 16: return fwd_dispatch<16>(p, io, nin, nf, s);
+    case 32: return fwd_dispatch<32>(p, io, nin, nf, s);
+    case 64: return fwd_dispatch<64>(p, io, nin, nf, s);
+    case 128: return fwd_dispatch<128>(p, io, nin, nf, s);
+    case 256: return fwd_dispatch<256>(p, io, nin, nf, s);
+    case 512: return fwd_dispatch<512>(p, io, nin, nf, s);
+    case 1024: return fwd_dispatch<1024>(p, io, nin, nf, s);
+    case 2048: return fwd_dispatch<2048>(p, io, nin, nf, s);
   }
   return fail(SHT_ERR_UNSUPPORTED, "im must be a power of two in [16, 2048]");
 }
@@ -360,8 +375,21 @@ int inv_typed(const Plan &p, const TF *four, const TF *four2, double *grid, doub
 }  // namespace
 
 int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s) {
-  if (f32) return fwd_typed(p, grid, static_cast<float *>(four), nf, s);
-  return fwd_typed(p, grid, static_cast<double *>(four), nf, s);
+  if (f32) {
+    FwdIO<float> io{{grid, nullptr}, {static_cast<float *>(four), nullptr}, {0, 0}};
+    return fwd_typed(p, io, 1, nf, s);
+  }
+  FwdIO<double> io{{grid, nullptr}, {static_cast<double *>(four), nullptr}, {0, 0}};
+  return fwd_typed(p, io, 1, nf, s);
+}
+
+int launch_fft_fwd_div(const Plan &p, const double *a, double *fa, const double *b, double *fb,
+                       int nf, hipStream_t s) {
+  FwdIO<double> io{};
+  int n = 0;
+  if (a) { io.src[n] = a; io.dst[n] = fa; io.mode[n] = 1; ++n; }
+  if (b) { io.src[n] = b; io.dst[n] = fb; io.mode[n] = 2; ++n; }
+  return fwd_typed(p, io, n, nf, s);
 }
 
 int launch_fft_inv(const Plan &p, const void *four, const void *four2, bool f32, double *grid,

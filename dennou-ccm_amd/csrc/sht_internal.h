@@ -2,6 +2,7 @@
 // the translation units of libsht_hip.so (gfx950 only).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -35,6 +36,7 @@ int fail(int code, const std::string &msg);
 //
 //   Pinv  per m: [par 2][KP(m)][P2p]  Pt_n^m(mu_p)          (inverse, p fastest)
 //   Hinv  per m: [par 2][KP(m)][P2p]  (1-mu^2) dPt/dmu      (inverse GradLat)
+//   Hfwd  per m: [par 2][P2p][KR(m)]  -(w_p/2) (1-mu^2) dPt/dmu (forward DivLat)
 //   Pfwd  per m: [par 2][P2p][KR(m)]  (w_p/2) Pt_n^m(mu_p)  (forward, k fastest)
 //   KP(m) = roundup(ceil((N-m+1)/2), 4), KR(m) = roundup(ceil((N-m+1)/2), 16),
 //   padding entries are zero.
@@ -59,6 +61,7 @@ struct Plan {
   double *twid = nullptr;                                // im complex: e^{2 pi i t/im}
   double *zero = nullptr;   // 64 zeros: the source of masked-off operand loads
   double *Pinv = nullptr, *Hinv = nullptr, *Pfwd = nullptr;
+  double *Hfwd = nullptr;  // -(w/2) (1-mu^2) dPt/dmu, Pfwd layout (divergence; GRAD plans)
   int *poff_inv = nullptr, *poff_fwd = nullptr;          // N+1 offsets (doubles)
   std::vector<int> h_poff_inv, h_poff_fwd;
   size_t n_inv = 0, n_fwd = 0;
@@ -117,6 +120,30 @@ __host__ __device__ inline int64_t coef_off(int N, int m) {
   return (int64_t)m * (N + 1) - (int64_t)m * (m - 1) / 2;
 }
 
+// ------------------------------------------------------------------ kernel launch
+// Stage timing: while a timer is open on the calling thread, transform kernels
+// are launched through hipExtLaunchKernel with events that the runtime records
+// at the kernel's own start (first launch of the stage) and end (every launch;
+// the last one stands), so a stage's time is its kernels' execution, not the
+// launch overhead that events recorded around the launch would add.
+struct StageTimer {
+  hipEvent_t start = nullptr, stop = nullptr;
+  bool used = false;
+};
+extern thread_local StageTimer *tl_stage_timer;
+
+template <typename... KArgs, typename... Args>
+inline void launch_kernel(void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem,
+                          hipStream_t s, Args... args) {
+  if (StageTimer *t = tl_stage_timer) {
+    hipExtLaunchKernelGGL(kernel, grid, block, (std::uint32_t)shmem, s, t->used ? nullptr : t->start,
+                          t->stop, 0u, args...);
+    t->used = true;
+  } else {
+    kernel<<<grid, block, shmem, s>>>(args...);
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 // Table construction (sht_tables.hip)
 int build_tables(Plan &p);
@@ -124,6 +151,10 @@ int build_tables(Plan &p);
 // FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos).
 // f32 = the Fourier intermediate is FP32 (mixed-precision plans); the FFT itself is FP64.
 int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s);
+// Divergence analysis FFT: a -> fa with x i m / cos, b -> fb with x 1 / cos (FP64;
+// either pair may be NULL), one launch.
+int launch_fft_fwd_div(const Plan &p, const double *a, double *fa, const double *b, double *fb,
+                       int nf, hipStream_t s);
 int launch_fft_inv(const Plan &p, const void *four, const void *four2, bool f32, double *grid,
                    double *gradlon, double *gradlat, int nf, hipStream_t s);
 
@@ -137,6 +168,12 @@ int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, doubl
 int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
                        bool value_table, bool deriv_table, int nf, hipStream_t s);
 int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s);
+// Forward Legendre (K-split shape) with any forward-layout table; swap = take the
+// opposite parity slot of the Fourier data (H tables), accumulate = add into spec.
+int launch_leg_fwd_f64_table(const Plan &p, const double *table, const double *four, double *spec,
+                             bool swap, bool accumulate, int nf, hipStream_t s);
+// spec_out = (-n(n+1))^power spec_in (negative powers leave n = 0 at zero)
+int launch_spec_lapla(const Plan &p, const double *in, double *out, int power, int nf, hipStream_t s);
 
 // Legendre stage, mixed precision (sht_legendre_mixed.hip)
 int build_mixed_tables(Plan &p);

@@ -114,6 +114,8 @@ __global__ __launch_bounds__(256) void leg_inv_f64_kernel(
 }
 
 // Forward: four (S, A halves) -> spec.  Workgroup = one (m, 16-column tile);
+// with swap the parity-q degrees read slot 1-q (tables of opposite parity,
+// e.g. the divergence H table), with accum the result is added to spec.
 // wave w takes the latitude-pair steps w, w+4, ... (4 pairs each) of the
 // quadrature, the partial sums over its pairs meet in LDS and are added in
 // FP64.  MAXRT = row tiles of 16 degrees per parity at m = 0.
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
                                                           const int *__restrict__ poff,
                                                           const int *__restrict__ mlist, int mlen,
                                                           int N, int P2, int P2p, int ncol, int nct,
-                                                          FourGeom geo) {
+                                                          FourGeom geo, int swap, int accum) {
   extern __shared__ double red[]; // [kWaves][MAXRT][4][64]
   int m, ct;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
@@ -157,7 +159,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
       const int pb = vb ? p : 0, pa = ok ? p : 0;
 #pragma unroll
       for (int q = 0; q < 2; ++q) {
-        const double x = four[geo.at(pb, m, q, csafe)];
+        const double x = four[geo.at(pb, m, q ^ swap, csafe)];
         bv[q][j] = vb ? x : 0.0;
 #pragma unroll
         for (int r = 0; r < MAXRT; ++r)
@@ -184,7 +186,10 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
       const double v = red[e] + red[e + MAXRT * 256] + red[e + 2 * MAXRT * 256] + red[e + 3 * MAXRT * 256];
       const int ln = e & 63, ri = e >> 6, i = ri & 3, r = ri >> 2;
       const int k = r * 16 + i * 4 + (ln >> 4), c = ct * 16 + (ln & 15);
-      if (k < kcnt[q] && c < ncol) sm[(size_t)(2 * k + q) * ncol + c] = v;
+      if (k < kcnt[q] && c < ncol) {
+        double *o = sm + (size_t)(2 * k + q) * ncol + c;
+        *o = accum ? *o + v : v;
+      }
     }
     __syncthreads();
   }
@@ -489,7 +494,7 @@ int inv_tile_launch(const Plan &p, KSplit ks, const double *spec, double *four, 
   const long nblk = ks.m2 == 0 ? xcd_m_blocks(p, 0, (int)w1)
                                : ks.m4 * w4 + (long)(ks.m2 - ks.m4) * w2 + (long)(p.N + 1 - ks.m2) * w1;
   const size_t lds = ks.m2 > 0 ? sizeof(double) * 4 * RT * 256 : 0;
-  leg_inv_f64_tile_kernel<RT, D><<<(int)nblk, 256, lds, s>>>(
+  launch_kernel(leg_inv_f64_tile_kernel<RT, D>, (int)nblk, 256, lds, s,
       spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.zero, p.N, p.P2, p.P2p, ncol, nct, tab0,
       ntab, ks, p.xcd_m[0], p.xcd_mlen[0], four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
@@ -501,7 +506,7 @@ int inv_wave_launch(const Plan &p, const double *spec, double *four, double *fou
                     int ntab, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int upm = ntab * (p.P2p / 16) * nct;
-  leg_inv_f64_wave_kernel<D><<<xcd_m_blocks(p, 0, (upm + 3) / 4), 256, 0, s>>>(
+  launch_kernel(leg_inv_f64_wave_kernel<D>, xcd_m_blocks(p, 0, (upm + 3) / 4), 256, 0, s,
       spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p,
       ncol, nct, tab0, ntab, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
@@ -640,7 +645,7 @@ int fwd_tile_launch(const Plan &p, KSplit ks, const double *four, double *spec, 
                                : (long)ks.m4 * nct + (long)(ks.m2 - ks.m4) * ((nct + 1) / 2) +
                                      (long)(p.N + 1 - ks.m2) * ((nct + 3) / 4);
   const size_t lds = ks.m2 > 0 ? sizeof(double) * 4 * MAXRT * 256 : 0;
-  leg_fwd_f64_tile_kernel<MAXRT, D><<<(int)nblk, 256, lds, s>>>(
+  launch_kernel(leg_fwd_f64_tile_kernel<MAXRT, D>, (int)nblk, 256, lds, s,
       four, spec, p.Pfwd, p.poff_fwd, p.zero, p.N, p.P2, p.P2p, ncol, nct, ks, p.xcd_m[1],
       p.xcd_mlen[1], four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
@@ -651,7 +656,7 @@ template <int D>
 int fwd_wave_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int nunits = p.n_fwd_units * nct;
-  leg_fwd_f64_wave_kernel<D><<<(nunits + 3) / 4, 256, 0, s>>>(
+  launch_kernel(leg_fwd_f64_wave_kernel<D>, (nunits + 3) / 4, 256, 0, s,
       four, spec, p.Pfwd, p.poff_fwd, p.fwd_units, p.N, p.P2, p.P2p, ncol, nct, nunits,
       four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
@@ -664,22 +669,22 @@ int inv_launch(const Plan &p, const double *spec, double *four, double *four2, i
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int nblk = xcd_m_blocks(p, 0, ntab * nct);
   const size_t lds = sizeof(double) * kWaves * RT * 256;
-  leg_inv_f64_kernel<RT><<<nblk, 256, lds, s>>>(spec, four, four2, p.Pinv, p.Hinv, p.poff_inv,
-                                                p.xcd_m[0], p.xcd_mlen[0], p.N,
-                                                p.P2, p.P2p, ncol, nct, tab0, ntab,
-                                                four_geom(p.N, p.im, nf));
+  launch_kernel(leg_inv_f64_kernel<RT>, nblk, 256, lds, s, spec, four, four2, p.Pinv, p.Hinv,
+                p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, tab0, ntab,
+                four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
 
 template <int MAXRT>
-int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
+int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s,
+               const double *table = nullptr, bool swap = false, bool accum = false) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int nblk = xcd_m_blocks(p, 1, nct);
   const size_t lds = sizeof(double) * kWaves * MAXRT * 256;
-  leg_fwd_f64_kernel<MAXRT><<<nblk, 256, lds, s>>>(four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1],
-                                                   p.xcd_mlen[1], p.N, p.P2,
-                                                   p.P2p, ncol, nct, four_geom(p.N, p.im, nf));
+  launch_kernel(leg_fwd_f64_kernel<MAXRT>, nblk, 256, lds, s, four, spec, table ? table : p.Pfwd,
+                p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
+                four_geom(p.N, p.im, nf), swap ? 1 : 0, accum ? 1 : 0);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -759,6 +764,21 @@ int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, doubl
     case 6: return fwd_launch<6>(p, four, spec, nf, s);
     case 7: return fwd_launch<7>(p, four, spec, nf, s);
     case 8: return fwd_launch<8>(p, four, spec, nf, s);
+  }
+  return fail(SHT_ERR_UNSUPPORTED, "nmax above 255 is not instantiated");
+}
+
+int launch_leg_fwd_f64_table(const Plan &p, const double *table, const double *four, double *spec,
+                             bool swap, bool accumulate, int nf, hipStream_t s) {
+  switch ((ceil_half(p.N + 1) + 15) / 16) {
+    case 1: return fwd_launch<1>(p, four, spec, nf, s, table, swap, accumulate);
+    case 2: return fwd_launch<2>(p, four, spec, nf, s, table, swap, accumulate);
+    case 3: return fwd_launch<3>(p, four, spec, nf, s, table, swap, accumulate);
+    case 4: return fwd_launch<4>(p, four, spec, nf, s, table, swap, accumulate);
+    case 5: return fwd_launch<5>(p, four, spec, nf, s, table, swap, accumulate);
+    case 6: return fwd_launch<6>(p, four, spec, nf, s, table, swap, accumulate);
+    case 7: return fwd_launch<7>(p, four, spec, nf, s, table, swap, accumulate);
+    case 8: return fwd_launch<8>(p, four, spec, nf, s, table, swap, accumulate);
   }
   return fail(SHT_ERR_UNSUPPORTED, "nmax above 255 is not instantiated");
 }

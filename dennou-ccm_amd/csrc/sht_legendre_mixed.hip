@@ -421,7 +421,7 @@ int build_mixed_mode(Plan &p) {
 template <int MODE, int NSL>
 int inv_nsl(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  leg_inv_mixed_kernel<MODE, NSL><<<xcd_m_blocks(p, 0, nct), 256, 0, s>>>(
+  launch_kernel(leg_inv_mixed_kernel<MODE, NSL>, xcd_m_blocks(p, 0, nct), 256, 0, s,
       spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
       p.xcd_mlen[0], p.N, p.P2, p.P2p,
       ncol, nct, four_geom(p.N, p.im, nf));
@@ -443,7 +443,7 @@ int inv_mode(const Plan &p, const double *spec, float *four, int nf, hipStream_t
 template <int MODE, int NSL>
 int fwd_nsl(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  leg_fwd_mixed_kernel<MODE, NSL><<<xcd_m_blocks(p, 1, nct), 256, 0, s>>>(
+  launch_kernel(leg_fwd_mixed_kernel<MODE, NSL>, xcd_m_blocks(p, 1, nct), 256, 0, s,
       four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
       p.xcd_mlen[1], p.N, p.P2,
       roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));

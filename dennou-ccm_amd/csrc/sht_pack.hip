@@ -1,7 +1,10 @@
 // sht_pack.hip -- spectral layout conversion between SPML's real (N+1)^2
 // packing (Fortran wa((N+1)**2, k), index l = n(n+1)+m+1 of SPML l_nm) and the
 // canonical complex m-major layout the Legendre GEMMs read, plus the global
-// mean AvrLonLat_xy (atm/dccm_atm_mod.f90:668-669, :774).
+// mean AvrLonLat_xy (atm/dccm_atm_mod.f90:668-669, :774) and the diagonal
+// spectral operators (SPML wa_Lapla_wa / wa_LaplaInv_wa and their powers, the
+// hyperdiffusion of DCPAM's dynamics namelist, HDOrder in
+// exp/APEI07Couple/common/dcpam/dcpam_APEI07Couple_T21L16.conf:205-230).
 #include "sht_internal.h"
 
 namespace sht {
@@ -73,7 +76,38 @@ __global__ __launch_bounds__(256) void avr_kernel(const double *__restrict__ gri
   if (threadIdx.x == 0) avg[f] = red[0] / (2.0 * im);
 }
 
+// one thread per (canonical k, field) complex pair: out = (-n(n+1))^power in
+__global__ void lapla_kernel(const double2 *__restrict__ in, double2 *__restrict__ out, int N,
+                             int nf, int power) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int K = (N + 1) * (N + 2) / 2;
+  if (t >= (int64_t)K * nf) return;
+  const int k = (int)(t / nf);
+  int m = 0;
+  while (m < N && coef_off(N, m + 1) <= k) ++m;
+  const int n = m + (k - (int)coef_off(N, m));
+  const double e = -(double)n * (n + 1);
+  double fac = 1.0;
+  if (power < 0 && n == 0) {
+    fac = 0.0;
+  } else {
+    const int ap = power < 0 ? -power : power;
+    for (int i = 0; i < ap; ++i) fac = power < 0 ? fac / e : fac * e;
+  }
+  const double2 v = in[t];
+  out[t] = make_double2(fac * v.x, fac * v.y);
+}
+
 }  // namespace
+
+int launch_spec_lapla(const Plan &p, const double *in, double *out, int power, int nf,
+                      hipStream_t s) {
+  const int64_t n = (int64_t)p.K * nf;
+  lapla_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+      reinterpret_cast<const double2 *>(in), reinterpret_cast<double2 *>(out), p.N, nf, power);
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
 
 int launch_spml_to_canon(const Plan &p, const double *wa, double *spec, int nf, hipStream_t s) {
   const int64_t n = (int64_t)p.K * nf;

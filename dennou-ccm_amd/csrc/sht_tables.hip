@@ -83,7 +83,7 @@ __global__ void legendre_kernel(int N, int P2, int P2p, const double *__restrict
                                 const double *__restrict__ coslat, const double *__restrict__ w,
                                 const int *__restrict__ poff_inv, const int *__restrict__ poff_fwd,
                                 double *__restrict__ Pinv, double *__restrict__ Pfwd,
-                                double *__restrict__ Hinv) {
+                                double *__restrict__ Hinv, double *__restrict__ Hfwd) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int p = t % P2, m = t / P2;
   if (m > N) return;
@@ -94,6 +94,7 @@ __global__ void legendre_kernel(int N, int P2, int P2p, const double *__restrict
   double *Pi = Pinv + poff_inv[m];
   double *Pf = Pfwd + poff_fwd[m];
   double *Hi = Hinv ? Hinv + poff_inv[m] : nullptr;
+  double *Hf = Hfwd ? Hfwd + poff_fwd[m] : nullptr;
 
   double pc = 1.0;
   int e = 0;
@@ -123,6 +124,8 @@ __global__ void legendre_kernel(int N, int P2, int P2p, const double *__restrict
       // (1-mu^2) dPt_n/dmu = (n+1) eps_n Pt_{n-1} - n eps_{n+1} Pt_{n+1}
       double h = (n > m ? (n + 1.0) * eps_nm(n, m) * pp : 0.0) - n * eps_nm(n + 1, m) * pn;
       Hi[(size_t)(par * KP + k) * P2p + p] = xout(h, e);
+      // divergence analysis weights -(w/2) H (the lat term integrated by parts)
+      Hf[(size_t)(par * P2p + p) * KR + k] = -hw * xout(h, e);
     }
     pp = pc;
     pc = pn;
@@ -165,6 +168,7 @@ int build_tables(Plan &p) {
   SHT_HIP(hipMalloc(&p.Pinv, sizeof(double) * ti));
   SHT_HIP(hipMalloc(&p.Pfwd, sizeof(double) * tf));
   if (p.flags & SHT_FLAG_GRAD) SHT_HIP(hipMalloc(&p.Hinv, sizeof(double) * ti));
+  if (p.flags & SHT_FLAG_GRAD) SHT_HIP(hipMalloc(&p.Hfwd, sizeof(double) * tf));
   SHT_HIP(hipMalloc(&p.poff_inv, sizeof(int) * (N + 1)));
   SHT_HIP(hipMalloc(&p.poff_fwd, sizeof(int) * (N + 1)));
   SHT_HIP(hipMemcpy(p.poff_inv, p.h_poff_inv.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
@@ -172,6 +176,7 @@ int build_tables(Plan &p) {
   SHT_HIP(hipMemset(p.Pinv, 0, sizeof(double) * ti));
   SHT_HIP(hipMemset(p.Pfwd, 0, sizeof(double) * tf));
   if (p.Hinv) SHT_HIP(hipMemset(p.Hinv, 0, sizeof(double) * ti));
+  if (p.Hfwd) SHT_HIP(hipMemset(p.Hfwd, 0, sizeof(double) * tf));
   std::vector<int> units;
   for (int m = 0; m <= N; ++m)
     for (int kt = 0; kt < roundup(ceil_half(N - m + 1), 16) / 16; ++kt) units.push_back(m << 8 | kt);
@@ -209,7 +214,7 @@ int build_tables(Plan &p) {
   SHT_HIP(hipGetLastError());
   int nthr = P2 * (N + 1);
   legendre_kernel<<<(nthr + 127) / 128, 128>>>(N, P2, P2p, p.mu, p.coslat, p.w, p.poff_inv,
-                                                 p.poff_fwd, p.Pinv, p.Pfwd, p.Hinv);
+                                                 p.poff_fwd, p.Pinv, p.Pfwd, p.Hinv, p.Hfwd);
   SHT_HIP(hipGetLastError());
   twiddle_kernel<<<(p.im + 127) / 128, 128>>>(p.im, p.twid);
   SHT_HIP(hipGetLastError());

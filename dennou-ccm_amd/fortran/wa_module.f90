@@ -18,6 +18,7 @@ module wa_module
   public :: wa_Initial, wa_Finalize
   public :: xya_wa, wa_xya, xya_GradLon_wa, xya_GradLat_wa
   public :: xy_wa, wa_xy
+  public :: wa_DivLon_xya, wa_DivLat_xya, wa_Lapla_wa, wa_LaplaInv_wa
   public :: l_nm, nm_l
   public :: AvrLonLat_xy, IntLonLat_xy
   public :: x_Lon, y_Lat, x_Lon_Weight, y_Lat_Weight, xy_Lon, xy_Lat
@@ -74,6 +75,34 @@ module wa_module
       type(c_ptr), value :: plan
       real(c_double), intent(in) :: wa(*)
       real(c_double), intent(out) :: xya(*)
+      integer(c_int), value :: nlev
+    end function
+    integer(c_int) function c_divlon(plan, xya, wa, nlev) bind(C, name="sht_wa_divlon_xya")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: xya(*)
+      real(c_double), intent(out) :: wa(*)
+      integer(c_int), value :: nlev
+    end function
+    integer(c_int) function c_divlat(plan, xya, wa, nlev) bind(C, name="sht_wa_divlat_xya")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: xya(*)
+      real(c_double), intent(out) :: wa(*)
+      integer(c_int), value :: nlev
+    end function
+    integer(c_int) function c_lapla(plan, wa, wout, nlev) bind(C, name="sht_wa_lapla_wa")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: wa(*)
+      real(c_double), intent(out) :: wout(*)
+      integer(c_int), value :: nlev
+    end function
+    integer(c_int) function c_laplainv(plan, wa, wout, nlev) bind(C, name="sht_wa_laplainv_wa")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: plan
+      real(c_double), intent(in) :: wa(*)
+      real(c_double), intent(out) :: wout(*)
       integer(c_int), value :: nlev
     end function
     integer(c_int) function c_l_nm(n, m) bind(C, name="sht_l_nm")
@@ -178,6 +207,34 @@ contains
     real(dp) :: w((nm + 1)**2)
     call check(c_wa_xya(plan, xy, w, 1_c_int), 'wa_xy')
   end function wa_xy
+
+  ! (1/cos lat) dA/dlon to spectra
+  function wa_DivLon_xya(xya) result(wa)
+    real(dp), intent(in) :: xya(0:, :, :)
+    real(dp) :: wa((nm + 1)**2, size(xya, 3))
+    call check(c_divlon(plan, xya, wa, int(size(xya, 3), c_int)), 'wa_DivLon_xya')
+  end function wa_DivLon_xya
+
+  ! (1/cos lat) d(A cos lat)/dlat to spectra
+  function wa_DivLat_xya(xya) result(wa)
+    real(dp), intent(in) :: xya(0:, :, :)
+    real(dp) :: wa((nm + 1)**2, size(xya, 3))
+    call check(c_divlat(plan, xya, wa, int(size(xya, 3), c_int)), 'wa_DivLat_xya')
+  end function wa_DivLat_xya
+
+  ! -n(n+1) wa (unit sphere)
+  function wa_Lapla_wa(wa) result(wout)
+    real(dp), intent(in) :: wa(:, :)
+    real(dp) :: wout(size(wa, 1), size(wa, 2))
+    call check(c_lapla(plan, wa, wout, int(size(wa, 2), c_int)), 'wa_Lapla_wa')
+  end function wa_Lapla_wa
+
+  ! wa / (-n(n+1)), zero for n = 0
+  function wa_LaplaInv_wa(wa) result(wout)
+    real(dp), intent(in) :: wa(:, :)
+    real(dp) :: wout(size(wa, 1), size(wa, 2))
+    call check(c_laplainv(plan, wa, wout, int(size(wa, 2), c_int)), 'wa_LaplaInv_wa')
+  end function wa_LaplaInv_wa
 
   integer function l_nm(n, m)
     integer, intent(in) :: n, m

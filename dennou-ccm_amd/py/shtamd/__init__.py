@@ -76,6 +76,8 @@ SIGNATURES = {
     "sht_spec_to_grid_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_grid_to_spec_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_spec_to_grad_dev": (_ci, [_vp, _vp, _vp, _vp, _vp, _ci, _vp]),
+    "sht_grid_to_div_dev": (_ci, [_vp, _vp, _vp, _vp, _ci, _vp]),
+    "sht_spec_lapla_dev": (_ci, [_vp, _vp, _vp, _ci, _ci, _vp]),
     "sht_spml_to_canon_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_canon_to_spml_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_avr_lonlat_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
@@ -83,6 +85,10 @@ SIGNATURES = {
     "sht_wa_xya": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_xya_gradlon_wa": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_xya_gradlat_wa": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_wa_divlon_xya": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_wa_divlat_xya": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_wa_lapla_wa": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_wa_laplainv_wa": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_batch_create": (_ci, [ctypes.POINTER(_MemberDesc), _ci, _ci, ctypes.POINTER(_vp)]),
     "sht_batch_destroy": (None, [_vp]),
     "sht_batch_member": (_ci, [_vp, _ci, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
@@ -193,6 +199,15 @@ class Plan:
         check(lib().sht_spec_to_grad_dev(self._h, _ptr(spec), _ptr(grid), _ptr(gradlon), _ptr(gradlat),
                                          nfield, _stream_handle(stream)))
 
+    def grid_to_div(self, a, b, spec, nfield: int, stream=None):
+        """Spectra of (1/cos) dA/dlon + (1/cos) d(B cos)/dlat; a or b may be None."""
+        check(lib().sht_grid_to_div_dev(self._h, _ptr(a), _ptr(b), _ptr(spec), nfield,
+                                        _stream_handle(stream)))
+
+    def spec_lapla(self, spec_in, spec_out, power: int, nfield: int, stream=None):
+        check(lib().sht_spec_lapla_dev(self._h, _ptr(spec_in), _ptr(spec_out), power, nfield,
+                                       _stream_handle(stream)))
+
     def spml_to_canon(self, wa, spec, nfield: int, stream=None):
         check(lib().sht_spml_to_canon_dev(self._h, _ptr(wa), _ptr(spec), nfield, _stream_handle(stream)))
 
@@ -249,6 +264,18 @@ class Plan:
 
     def xya_gradlat_wa(self, wa):
         return self._host_call(lib().sht_xya_gradlat_wa, wa, (wa.shape[0], self.jm, self.im))
+
+    def wa_divlon_xya(self, xya):
+        return self._host_call(lib().sht_wa_divlon_xya, xya, (xya.shape[0], (self.nmax + 1) ** 2))
+
+    def wa_divlat_xya(self, xya):
+        return self._host_call(lib().sht_wa_divlat_xya, xya, (xya.shape[0], (self.nmax + 1) ** 2))
+
+    def wa_lapla_wa(self, wa):
+        return self._host_call(lib().sht_wa_lapla_wa, wa, wa.shape)
+
+    def wa_laplainv_wa(self, wa):
+        return self._host_call(lib().sht_wa_laplainv_wa, wa, wa.shape)
 
 
 class Batch:

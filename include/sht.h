@@ -110,6 +110,20 @@ int sht_grid_to_spec_dev(sht_plan *plan, const double *grid, double *spec,
  * any of the three outputs may be NULL.  GradLat needs SHT_FLAG_GRAD. */
 int sht_spec_to_grad_dev(sht_plan *plan, const double *spec, double *grid,
                          double *gradlon, double *gradlat, int nfield, void *stream);
+/* wa_DivLon_xya, wa_DivLat_xya and their sum (the flux divergence of DCPAM's
+ * dynamics, atm/dcpam_main_mod.f90:2173-2180): spectra of
+ *   (1/cos lat) dA/dlon + (1/cos lat) d(B cos lat)/dlat
+ * for grid fields A and B, either of which may be NULL.  The lat term is the
+ * quadrature of -B (1-mu^2) dPt/dmu / cos lat (integration by parts, exact
+ * for band-limited input); it needs SHT_FLAG_GRAD.  FP64 in every form. */
+int sht_grid_to_div_dev(sht_plan *plan, const double *a, const double *b, double *spec,
+                        int nfield, void *stream);
+/* wa_Lapla_wa (power 1), wa_LaplaInv_wa (power -1) and their powers (the
+ * spectral hyperdiffusion, HDOrder in dcpam_APEI07Couple_T21L16.conf:205-230):
+ * out = (-n(n+1))^power in on canonical spectra, power in [-4, 4]; negative
+ * powers leave n = 0 at zero.  in == out is allowed. */
+int sht_spec_lapla_dev(const sht_plan *plan, const double *in, double *out, int power,
+                       int nfield, void *stream);
 /* SPML real packing <-> canonical complex layout */
 int sht_spml_to_canon_dev(const sht_plan *plan, const double *wa, double *spec,
                           int nfield, void *stream);
@@ -125,6 +139,10 @@ int sht_xya_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
 int sht_wa_xya(sht_plan *plan, const double *xya, double *wa, int nlev);
 int sht_xya_gradlon_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
 int sht_xya_gradlat_wa(sht_plan *plan, const double *wa, double *xya, int nlev);
+int sht_wa_divlon_xya(sht_plan *plan, const double *xya, double *wa, int nlev);
+int sht_wa_divlat_xya(sht_plan *plan, const double *xya, double *wa, int nlev);  /* SHT_FLAG_GRAD */
+int sht_wa_lapla_wa(sht_plan *plan, const double *wa, double *wa_out, int nlev);     /* on the host */
+int sht_wa_laplainv_wa(sht_plan *plan, const double *wa, double *wa_out, int nlev);  /* on the host */
 
 /* ---- ensemble batches: many members of uneven truncation and level count
  *      transformed by one hipGraph launch.  There is no SPML counterpart (SPML

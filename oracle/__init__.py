@@ -43,6 +43,8 @@ def lib():
         L.orc_spml_pack.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_spml_unpack.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.orc_avr_lonlat.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
+        L.orc_grid_to_div.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
+        L.orc_spec_lapla.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -126,11 +128,29 @@ class Plan:
         lib().orc_grid_to_spec(self._h, nf, _p(grid), _p(out.view(np.float64).reshape(-1)), nthreads)
         return out
 
+    def grid_to_div(self, a=None, b=None, nthreads: int = 0):
+        """Spectra of (1/cos) dA/dlon + (1/cos) d(B cos)/dlat; A or B may be None."""
+        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (a, b)]
+        nf = next(x for x in arrs if x is not None).shape[0]
+        out = np.empty((self.K, nf), dtype=np.complex128)
+        ptrs = [None if x is None else _p(x) for x in arrs]
+        lib().orc_grid_to_div(self._h, nf, ptrs[0], ptrs[1], _p(out.view(np.float64).reshape(-1)), nthreads)
+        return out
+
     def avr_lonlat(self, grid):
         grid = np.ascontiguousarray(grid, dtype=np.float64)
         out = np.empty(grid.shape[0])
         lib().orc_avr_lonlat(self._h, grid.shape[0], _p(grid), _p(out))
         return out
+
+
+def spec_lapla(spec, nmax: int, power: int):
+    """(-n(n+1))^power applied to canonical spectra (negative powers: n = 0 -> 0)."""
+    spec = np.ascontiguousarray(np.asarray(spec, dtype=np.complex128))
+    out = np.empty_like(spec)
+    lib().orc_spec_lapla(nmax, spec.shape[1], _p(spec.view(np.float64).reshape(-1)),
+                         _p(out.view(np.float64).reshape(-1)), power)
+    return out
 
 
 def spml_pack(spec, nmax: int):

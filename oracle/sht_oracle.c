@@ -466,6 +466,114 @@ void orc_grid_to_spec(const orc_plan *p, int nf, const double *grid, double *spe
   free(four);
 }
 
+/* grid -> spectral of the flux divergence
+ *   (1/cos phi) dA/dlon + (1/cos phi) d(B cos phi)/dphi
+ * (SPML wa_DivLon_xya for A alone, wa_DivLat_xya for B alone).  The lon term
+ * is i m A^m / cos phi, analysed like any field; the lat term is integrated by
+ * parts over the sphere (the boundary term vanishes with cos phi at the poles):
+ *   s_n^m = -sum_q (w_q/2) H_n^m(mu_q) B^m(mu_q) / cos phi_q,
+ *   H = (1-mu^2) dPt/dmu,
+ * whose integrand is a polynomial for band-limited B, so the Gauss sum is exact.
+ * H(-mu) = -(-1)^(n-m) H(mu): an even n-m pairs with the antisymmetric part of
+ * B^m over the mirrored latitudes, an odd n-m with the symmetric part. */
+void orc_grid_to_div(const orc_plan *p, int nf, const double *ga, const double *gb,
+                     double *spec, int nthreads) {
+  const int N = p->N, jm = p->jm, P2 = p->P2, im = p->im, M1 = N + 1;
+  const double *g[2] = {ga, gb};
+  double *four[2] = {NULL, NULL};
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+  nthreads = 1;
+#endif
+  for (int t = 0; t < 2; ++t) {
+    if (!g[t]) continue;
+    four[t] = (double *)malloc(sizeof(double) * (size_t)nf * jm * M1 * 2);
+    const double *grid = g[t];
+    double *fo = four[t];
+#pragma omp parallel num_threads(nthreads)
+    {
+      double *wb = (double *)malloc(sizeof(double) * 2 * im);
+#pragma omp for schedule(static)
+      for (long u = 0; u < (long)nf * P2; ++u) {
+        int f = (int)(u / P2), q = (int)(u % P2);
+        int jn = P2 + q, js = P2 - 1 - q;
+        ring_pair_fwd_any(p, grid + ((size_t)f * jm + jn) * im, grid + ((size_t)f * jm + js) * im,
+                          fo + ((size_t)f * jm + jn) * M1 * 2, fo + ((size_t)f * jm + js) * M1 * 2,
+                          wb, wb + im);
+      }
+      free(wb);
+    }
+  }
+#pragma omp parallel num_threads(nthreads)
+  {
+    /* per pair q: [A sym, A asy, B sym, B asy] x 2 nf */
+    double *sa = (double *)malloc(sizeof(double) * (size_t)P2 * 8 * nf);
+#pragma omp for schedule(dynamic, 1)
+    for (int m = 0; m <= N; ++m) {
+      for (int q = 0; q < P2; ++q) {
+        int jn = P2 + q, js = P2 - 1 - q;
+        double rc = 1.0 / p->coslat[jn];
+        double *blk = sa + (size_t)q * 8 * nf;
+        for (int t = 0; t < 2; ++t) {
+          double *sym = blk + (size_t)t * 4 * nf, *asy = sym + 2 * nf;
+          for (int f = 0; f < nf; ++f) {
+            double nr = 0, ni = 0, sr = 0, si = 0;
+            if (four[t]) {
+              const double *fn = four[t] + (((size_t)f * jm + jn) * M1 + m) * 2;
+              const double *fs = four[t] + (((size_t)f * jm + js) * M1 + m) * 2;
+              nr = fn[0] * rc; ni = fn[1] * rc; sr = fs[0] * rc; si = fs[1] * rc;
+              if (t == 0) { /* times i m */
+                double x;
+                x = nr; nr = -m * ni; ni = m * x;
+                x = sr; sr = -m * si; si = m * x;
+              }
+            }
+            sym[2 * f] = nr + sr; sym[2 * f + 1] = ni + si;
+            asy[2 * f] = nr - sr; asy[2 * f + 1] = ni - si;
+          }
+        }
+      }
+      double *s_m = spec + coef_off(N, m) * nf * 2;
+      for (int n = m; n <= N; ++n) {
+        double *sv = s_m + (size_t)(n - m) * nf * 2;
+        memset(sv, 0, sizeof(double) * 2 * nf);
+        int par = (n - m) & 1;
+        for (int q = 0; q < P2; ++q) {
+          double hw = 0.5 * p->w[P2 + q];
+          double tp = hw * p->P[p->poff[m] + (size_t)(n - m) * P2 + q];
+          double th = -hw * p->H[p->hoff[m] + (size_t)(n - m) * P2 + q];
+          const double *blk = sa + (size_t)q * 8 * nf;
+          const double *a = blk + par * 2 * nf;             /* A: same parity      */
+          const double *b = blk + 4 * nf + (1 - par) * 2 * nf; /* B: opposite parity */
+          for (int f = 0; f < 2 * nf; ++f) sv[f] += tp * a[f] + th * b[f];
+        }
+        if (m == 0)
+          for (int f = 0; f < nf; ++f) sv[2 * f + 1] = 0.0;
+      }
+    }
+    free(sa);
+  }
+  free(four[0]);
+  free(four[1]);
+}
+
+/* Powers of the unit-sphere Laplacian on canonical spectra: out = (-n(n+1))^power in
+ * (SPML wa_Lapla_wa: power 1; wa_LaplaInv_wa: power -1).  Negative powers set
+ * n = 0 to zero.  in and out may alias. */
+void orc_spec_lapla(int nmax, int nf, const double *in, double *out, int power) {
+  for (int m = 0; m <= nmax; ++m)
+    for (int n = m; n <= nmax; ++n) {
+      const double e = -(double)n * (n + 1);
+      double fac = 1.0;
+      if (power < 0 && n == 0) fac = 0.0;
+      else
+        for (int k = 0; k < (power < 0 ? -power : power); ++k) fac = power < 0 ? fac / e : fac * e;
+      const size_t o = (size_t)(coef_off(nmax, m) + n - m) * nf * 2;
+      for (int f = 0; f < 2 * nf; ++f) out[o + f] = fac * in[o + f];
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 
 void orc_spml_pack(int nmax, int nf, const double *spec, double *wa) {

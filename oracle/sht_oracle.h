@@ -66,6 +66,13 @@ void orc_spec_to_grid(const orc_plan *p, int nf, const double *spec,
 void orc_grid_to_spec(const orc_plan *p, int nf, const double *grid,
                       double *spec, int nthreads);
 
+/* grid -> spectral of (1/cos phi) dA/dlon + (1/cos phi) d(B cos phi)/dphi
+ * (SPML wa_DivLon_xya, wa_DivLat_xya and their sum); A or B may be NULL. */
+void orc_grid_to_div(const orc_plan *p, int nf, const double *a, const double *b,
+                     double *spec, int nthreads);
+/* out = (-n(n+1))^power in on canonical spectra (negative powers: n = 0 -> 0) */
+void orc_spec_lapla(int nmax, int nf, const double *in, double *out, int power);
+
 /* Canonical complex <-> real (N+1)^2 packing wa[f][l], l = n(n+1)+m,
  * m = -n..n; wa(n,0) = Re s, wa(n,m>0) = sqrt2 Re s, wa(n,-m) = -sqrt2 Im s
  * (orthonormal real harmonics with unit mean square). */

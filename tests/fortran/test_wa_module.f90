@@ -72,9 +72,35 @@ program test_wa_module
   if (err > 1e-12_dp) call bad('round trip')
   if (abs(AvrLonLat_xy(xya(:, :, 5)) - wa2(l_nm(0, 0), 5)) > 1e-13_dp) call bad('global mean')
 
+  ! flux divergence: DivLat(cos lat) = -2 sin lat = -(2/sqrt3) harmonic (1, 0);
+  ! div(grad f) = Laplacian f; LaplaInv(Lapla f) = f without its mean
+  do k = 1, km
+    do j = 1, jm
+      xya(:, j, k) = cl(j)
+    end do
+  end do
+  wa = wa_DivLat_xya(xya)
+  wa2 = 0.0_dp
+  wa2(l_nm(1, 0), :) = -2.0_dp / s3
+  err = maxval(abs(wa - wa2))
+  if (err > 1e-13_dp) call bad('wa_DivLat_xya closed form')
+  do k = 1, km
+    do i = 1, (nm+1)**2
+      pair = nm_l(i)
+      wa2(i, k) = cos(0.21_dp * i + 0.7_dp * k) / (1.0_dp + pair(1))**1.5_dp
+    end do
+  end do
+  wa = wa_DivLon_xya(xya_GradLon_wa(wa2)) + wa_DivLat_xya(xya_GradLat_wa(wa2))
+  err = maxval(abs(wa - wa_Lapla_wa(wa2))) / maxval(abs(wa_Lapla_wa(wa2)))
+  if (err > 1e-12_dp) call bad('div grad = Laplacian')
+  wa = wa_LaplaInv_wa(wa_Lapla_wa(wa2))
+  wa2(l_nm(0, 0), :) = 0.0_dp
+  err = maxval(abs(wa - wa2))
+  if (err > 1e-13_dp) call bad('LaplaInv(Lapla)')
+
   call wa_Finalize()
   if (nbad == 0) then
-    print '(a)', 'PASS wa_module drop-in (T21, 64x32, 16 levels)'
+    print '(a)', 'PASS wa_module drop-in (T21, 64x32, 16 levels; transforms, gradients, divergence, Laplacian)'
   else
     print '(a, i0, a)', 'FAIL: ', nbad, ' checks'
     error stop 1

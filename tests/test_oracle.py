@@ -375,3 +375,67 @@ def test_spml_packing_orthonormal_real_basis():
     # Parseval in the real packing: mean square = sum wa^2
     g = p.spec_to_grid(s)
     assert np.abs(p.avr_lonlat(g ** 2) - (oracle.spml_pack(s, N) ** 2).sum(1)).max() < 1e-14
+
+
+# ---- flux divergence and Laplacian powers (wa_DivLon_xya, wa_DivLat_xya,
+#      wa_Lapla_wa, wa_LaplaInv_wa; DCPAM dynamics, atm/dcpam_main_mod.f90:2173-2180)
+
+def _cidx(N, n, m):
+    return m * (N + 1) - m * (m - 1) // 2 + n - m
+
+
+def test_divergence_closed_forms():
+    """DivLat(cos lat) = -2 sin lat = -(2/sqrt3) Pt_1^0; DivLon(cos^2 lat sin lon) =
+    cos lat cos lon = Pt_1^1 cos lon / sqrt(3/2), i.e. s_1^1 = 1/sqrt6."""
+    N, im, jm = 21, 64, 32
+    p = oracle.Plan(N, im, jm)
+    mu, w, cl = oracle.gauss(jm)
+    lon = 2 * np.pi * np.arange(im) / im
+    s = p.grid_to_div(None, np.broadcast_to(cl[:, None], (jm, im))[None].copy())
+    want = np.zeros_like(s)
+    want[_cidx(N, 1, 0), 0] = -2 / np.sqrt(3)
+    assert np.abs(s - want).max() < 1e-14
+    s = p.grid_to_div((cl[:, None] ** 2 * np.sin(lon)[None, :])[None].copy(), None)
+    want = np.zeros_like(s)
+    want[_cidx(N, 1, 1), 0] = 1 / np.sqrt(6)
+    assert np.abs(s - want).max() < 1e-14
+
+
+@pytest.mark.parametrize("N,im,jm", [(5, 16, 8), (21, 64, 32), (42, 128, 64), (85, 256, 128)])
+def test_divergence_of_gradient_is_laplacian(N, im, jm):
+    """div(grad f) = -n(n+1) f_n exactly on the Gauss grid: the lon and lat
+    integrands are polynomials of degree < 2 jm for band-limited f."""
+    p = oracle.Plan(N, im, jm)
+    spec = synth.spectra(N, 3, cfg=8)
+    gl, gt = p.spec_to_grid(spec, kind=1), p.spec_to_grid(spec, kind=2)
+    lap = oracle.spec_lapla(spec, N, 1)
+    assert rel(p.grid_to_div(gl, gt), lap) < 1e-13
+    # each term alone is exact too: div_lon(grad_lon f) + div_lat(grad_lat f)
+    assert rel(p.grid_to_div(gl, None) + p.grid_to_div(None, gt), lap) < 1e-13
+
+
+def test_rotational_flow_vorticity_and_divergence():
+    """u = -dpsi/dlat, v = (1/cos) dpsi/dlon: divergence 0, vorticity
+    (1/cos) dv/dlon - (1/cos) d(u cos)/dlat = Laplacian of psi."""
+    N, im, jm = 42, 128, 64
+    p = oracle.Plan(N, im, jm)
+    psi = synth.spectra(N, 4, cfg=9)
+    u, v = -p.spec_to_grid(psi, kind=2), p.spec_to_grid(psi, kind=1)
+    lap = oracle.spec_lapla(psi, N, 1)
+    scale = np.abs(lap).max()
+    assert np.abs(p.grid_to_div(u, v)).max() < 1e-13 * scale
+    assert rel(p.grid_to_div(v, -u), lap) < 1e-13
+
+
+def test_laplacian_powers():
+    N = 21
+    spec = synth.spectra(N, 2, cfg=10)
+    n = np.concatenate([np.arange(m, N + 1) for m in range(N + 1)])
+    e = -(n * (n + 1.0))
+    for power in (1, 2, 4):
+        assert np.allclose(oracle.spec_lapla(spec, N, power), spec * (e ** power)[:, None], rtol=1e-15, atol=0)
+    inv = oracle.spec_lapla(oracle.spec_lapla(spec, N, 1), N, -1)
+    want = spec.copy()
+    want[0] = 0.0
+    assert rel(inv, want) < 1e-15
+    assert np.all(oracle.spec_lapla(spec, N, -2)[0] == 0.0)
