@@ -80,6 +80,7 @@ def run_gpu(precision: int, steps: int, warmup: int):
     dev = torch.device("cuda:0")
     spec_c = synth.state_spectra(N, NLEV)
     plan = shtamd.Plan(N, IM, JM, NF, precision=precision)
+    shapes, _ = plan.autotune(NF, reps=20)  # Legendre kernel shapes for this batch
     s_in = torch.from_numpy(shtamd.spec_to_real(spec_c)).to(dev)
     grid = torch.empty((NF, JM, IM), dtype=torch.float64, device=dev)
     s_out = torch.empty_like(s_in)
@@ -152,7 +153,7 @@ def run_gpu(precision: int, steps: int, warmup: int):
     kernel_ms = {k: (v[0] / v[1] if v[1] else None) for k, v in st.items()}
     plan.close()
     return {"eager_ms": eager_ms, "graph_ms": graph_ms, "kernel_ms": kernel_ms,
-            "grid": out_grid, "spec": out_spec, "spec_in": spec_c}
+            "grid": out_grid, "spec": out_spec, "spec_in": spec_c, "shapes": shapes}
 
 
 def cpu_baseline(spec_c, seconds: float, threads: int):
@@ -231,6 +232,17 @@ def traffic_from_profile(form: str, kernel_prefix: str):
 
 
 KERNEL_NAMES = {"fft_inv": "fft_inv_kernel", "fft_fwd": "fft_fwd_kernel"}
+SHAPE_NAMES = ("ksplit", "wave", "tile", "tile_split")
+
+
+def legendre_kernel_name(stage: str, fp64: bool, shape: int) -> str:
+    """Kernel name (as rocprof prints it, up to the template list) of a
+    Legendre stage in a given shape."""
+    if fp64:
+        suffix = ("_f64_kernel", "_f64_wave_kernel", "_f64_tile_kernel", "_f64_tile_kernel")[shape]
+    else:
+        suffix = ("_mixed_kernel", "_mixed_wave_kernel")[shape]
+    return stage + suffix
 
 ENSEMBLE_GRIDS = {42: (128, 64), 85: (256, 128), 170: (512, 256)}
 
@@ -256,6 +268,8 @@ def run_ensemble(precision: str, nmember: int, steps: int, warmup: int, seed: in
         spec_in, _, _, ld = b.member_ptrs(j)
         hipmem.put_member_spectra(spec_in, ld, s)
         specs.append(s)
+    # per-group Legendre shapes from the autotuner, then the graph is captured
+    b.autotune(reps=5)
     stream = torch.cuda.current_stream()
     for _ in range(max(1, warmup)):
         b.run(shtamd.DIR_ROUNDTRIP, stream)
@@ -351,8 +365,13 @@ def main():
     dom = max(kms, key=kms.get)
     bytes_, flops = alg[dom]
     achieved = bytes_ / (kms[dom] * 1e-3) / 1e9
-    kname = KERNEL_NAMES.get(dom, dom + ("_f64_kernel" if prec == 0 else "_mixed_kernel"))
-    traffic = traffic_from_profile(headline, kname)
+    hshapes = results[headline]["shapes"]
+    if dom in KERNEL_NAMES:
+        kname = f"{KERNEL_NAMES[dom]}<{IM},{'double' if prec == 0 else 'float'}>"
+        traffic = traffic_from_profile(headline, kname)
+    else:
+        kname = legendre_kernel_name(dom, prec == 0, hshapes[0 if dom == "leg_inv" else 1])
+        traffic = traffic_from_profile(headline, kname + "<")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kname,
                 "kernel_us": round(kms[dom] * 1e3, 2), "algorithmic_bytes": int(bytes_)}
@@ -361,7 +380,8 @@ def main():
         t = {"fields_per_s": round(NF / (r["best_ms"] * 1e-3), 1), "ms_per_step": round(r["best_ms"], 5),
              "eager_ms": round(r["eager_ms"], 5),
              "graph_ms": None if r["graph_ms"] is None else round(r["graph_ms"], 5),
-             "err_vs_fp64_oracle": r["err"], "kernels_us": {}}
+             "err_vs_fp64_oracle": r["err"], "kernels_us": {},
+             "legendre_shapes": {"leg_inv": SHAPE_NAMES[r["shapes"][0]], "leg_fwd": SHAPE_NAMES[r["shapes"][1]]}}
         a = algorithmic(table_bytes(PRECISIONS[form]), NF, 16 if form == "fp64" else 8)
         for k, ms in r["kernel_ms"].items():
             if not ms:

@@ -213,6 +213,22 @@ int sht_batch_run(sht_batch *b, int dir, void *stream) {
   return SHT_OK;
 }
 
+int sht_batch_autotune(sht_batch *b, int reps, const char *cache_path) {
+  if (!b) return fail(SHT_ERR_INVALID, "sht_batch_autotune: batch is NULL");
+  for (auto &g : b->groups) {
+    int rc = sht_plan_autotune(g.plan, g.nf, reps, cache_path, nullptr);
+    if (rc) return rc;
+  }
+  // graphs captured before hold the old kernels: capture again on the next run
+  for (int d = 0; d < 3; ++d) {
+    if (b->exec[d]) (void)hipGraphExecDestroy(b->exec[d]);
+    if (b->graph[d]) (void)hipGraphDestroy(b->graph[d]);
+    b->exec[d] = nullptr;
+    b->graph[d] = nullptr;
+  }
+  return SHT_OK;
+}
+
 int sht_batch_run_eager(sht_batch *b, int dir, void *stream) {
   if (!b) return fail(SHT_ERR_INVALID, "sht_batch_run_eager: batch is NULL");
   if (dir < SHT_DIR_INVERSE || dir > SHT_DIR_ROUNDTRIP)

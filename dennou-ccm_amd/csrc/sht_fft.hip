@@ -25,11 +25,15 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kLdsComplex = 2048;
-// LDS holds one pad element per 16 complex values: the radix-16 pass writes
+// LDS holds one pad element per 16 complex values -- the radix-16 pass writes
 // 16 consecutive outputs per thread (a 256-byte lane stride), which without the
-// pad lands every lane of a ds_write on the same banks.
-constexpr int kLdsPadded = kLdsComplex + kLdsComplex / 16;
-__device__ __forceinline__ int px(int i) { return i + (i >> 4); }
+// pad lands every lane of a ds_write on the same banks -- and one more per
+// transform, because the Fourier-side loops run the G transforms of a block in
+// consecutive lanes (element m of transform g), a stride of L(1 + 1/16)
+// elements that is otherwise a multiple of the bank count.
+constexpr int kLdsPadded = kLdsComplex + kLdsComplex / 16 + kLdsComplex / 16;
+template <int L>
+__device__ __forceinline__ int px(int i) { return i + (i >> 4) + i / L; }
 
 __device__ __forceinline__ void store_pair(double *o, double2 v) { *reinterpret_cast<double2 *>(o) = v; }
 __device__ __forceinline__ void store_pair(float *o, double2 v) {
@@ -106,7 +110,7 @@ __device__ __forceinline__ void stockham_pass(double2 *buf, const double2 *tw) {
     if (NB % kThreads != 0 && idx >= NB) continue;
     const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[q][r] = buf[px(g * L + j + r * (L / R))];
+    for (int r = 0; r < R; ++r) v[q][r] = buf[px<L>(g * L + j + r * (L / R))];
     if constexpr (Ns > 1) {
       double2 w1 = tw[k * (L / (Ns * R))];
       w1.y *= SIGN;
@@ -127,7 +131,7 @@ __device__ __forceinline__ void stockham_pass(double2 *buf, const double2 *tw) {
     const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
     const int d0 = g * L + (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[px(d0 + r * Ns)] = v[q][r];
+    for (int r = 0; r < R; ++r) buf[px<L>(d0 + r * Ns)] = v[q][r];
   }
   __syncthreads();
 }
@@ -209,15 +213,15 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
   }
   for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) buf[px(tid + q * kThreads)] = z[q];
+  for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = z[q];
   __syncthreads();
   fft_lds<L, -1>(buf, twl);
   const double sc = 0.5 / L * (mode == 0 ? 1.0 : 1.0 / coslat[jn]);
   for (int idx = tid; idx < M1 * G; idx += kThreads) {
     const int m = idx / G, g = idx % G, f = f0 + g;
     if (f >= nf) continue;
-    const double2 zm = buf[px(g * L + m)];
-    const double2 zc = buf[px(g * L + ((L - m) & (L - 1)))];
+    const double2 zm = buf[px<L>(g * L + m)];
+    const double2 zc = buf[px<L>(g * L + ((L - m) & (L - 1)))];
     // X_N = (Z_m + conj Z_{L-m}) sc ;  X_S = -i (Z_m - conj Z_{L-m}) sc
     const double2 xn = make_double2((zm.x + zc.x) * sc, (zm.y - zc.y) * sc);
     const double2 xs = make_double2((zm.y + zc.y) * sc, -(zm.x - zc.x) * sc);
@@ -277,7 +281,7 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
     }
   }
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) buf[px(tid + q * kThreads)] = make_double2(0.0, 0.0);
+  for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = make_double2(0.0, 0.0);
   for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
   __syncthreads();
 #pragma unroll
@@ -295,10 +299,10 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
       xs = make_double2(xs.x * rc, xs.y * rc);
     }
     if (m == 0) {
-      buf[px(g * L)] = make_double2(xn.x, xs.x);
+      buf[px<L>(g * L)] = make_double2(xn.x, xs.x);
     } else {
-      buf[px(g * L + m)] = make_double2(xn.x - xs.y, xn.y + xs.x);
-      buf[px(g * L + L - m)] = make_double2(xn.x + xs.y, -xn.y + xs.x);
+      buf[px<L>(g * L + m)] = make_double2(xn.x - xs.y, xn.y + xs.x);
+      buf[px<L>(g * L + L - m)] = make_double2(xn.x + xs.y, -xn.y + xs.x);
     }
   }
   __syncthreads();
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
     const int idx = tid + q * kThreads;
     const int g = idx / L, i = idx % L, f = f0 + g;
     if (f >= nf) continue;
-    const double2 z = buf[px(idx)];
+    const double2 z = buf[px<L>(idx)];
     __builtin_nontemporal_store(z.x, dst + ((size_t)f * jm + jn) * L + i);
     __builtin_nontemporal_store(z.y, dst + ((size_t)f * jm + js) * L + i);
   }

@@ -178,6 +178,10 @@ int launch_spec_lapla(const Plan &p, const double *in, double *out, int power, i
 // Legendre stage, mixed precision (sht_legendre_mixed.hip)
 int build_mixed_tables(Plan &p);
 int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s);
+int launch_leg_inv_mixed_shape(const Plan &p, int shape, const double *spec, float *four, int nf,
+                               hipStream_t s);
+int launch_leg_fwd_mixed_shape(const Plan &p, int shape, const float *four, double *spec, int nf,
+                               hipStream_t s);
 int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s);
 
 // Layout helpers (sht_pack.hip)

@@ -331,6 +331,178 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
   }
 }
 
+// Wave shape: each wave owns one 16-column tile of an m and holds that tile's
+// data, split into pieces, in registers as the B operand: lane l reads the 8
+// values K = 32 s + 8 (l/16) .. +7 of column l % 16 and splits them itself
+// (the FP16 exponent is a shuffle maximum over the 4 lanes of a column), so
+// there is no LDS staging and no barrier.  The wave then runs every row tile
+// (inverse: 16 latitude pairs; forward: parity x 16 degrees), loading the
+// table fragments, which the four waves of a workgroup share through L1.
+template <int MODE>
+__device__ __forceinline__ void split8(const double (&v)[8], int e, uint4 (&pb)[3]) {
+  uint32_t w[3][4];
+#pragma unroll
+  for (int i = 0; i < 8; i += 2) {
+    uint16_t p0[3], p1[3];
+    split<MODE>(Form<MODE>::kScaled ? ldexp(v[i], e) : v[i], p0);
+    split<MODE>(Form<MODE>::kScaled ? ldexp(v[i + 1], e) : v[i + 1], p1);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) w[q][i >> 1] = p0[q] | ((uint32_t)p1[q] << 16);
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) pb[q] = make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
+}
+
+template <int MODE>
+__device__ __forceinline__ int col_exponent(const double (&v)[8]) {
+  if constexpr (!Form<MODE>::kScaled) return 0;
+  double mx = 0.0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) mx = fmax(mx, fabs(v[i]));
+  mx = fmax(mx, __shfl_xor(mx, 16));
+  mx = fmax(mx, __shfl_xor(mx, 32));
+  return mx > 0.0 ? 13 - ilogb(mx) : 0;
+}
+
+template <int MODE, int NSL>
+__global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
+    const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
+    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
+    int mlen, const double *__restrict__ zero, int N, int P2, int P2p, int ncol, int nct,
+    FourGeom geo) {
+  constexpr int NP = Form<MODE>::NP;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + 3) >> 2, m, t)) return;
+  const int ct = t * 4 + wave;
+  if (ct >= nct) return;
+  const int M1 = N + 1;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
+  const int col = ct * 16 + r16;
+  const bool cok = col < ncol;
+  const double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + (cok ? col : 0);
+  const uint16_t *__restrict__ Lm = L + loff[m];
+  uint4 pb[2][NSL][3];
+  double bsc[2][NSL];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int sl = 0; sl < NSL; ++sl) {
+      double v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int k = sl * 32 + kq * 8 + i;
+        const bool ok = sl < nsl && cok && k < (q ? ko : ke);
+        v[i] = *(ok ? sm + (size_t)(2 * k + q) * ncol : zero);
+      }
+      const int e = col_exponent<MODE>(v);
+      bsc[q][sl] = ldexp(1.0, -e);
+      split8<MODE>(v, e, pb[q][sl]);
+    }
+  const int rtiles = P2p >> 4;
+  for (int rt = 0; rt < rtiles; ++rt) {
+    uint4 pa[2][NSL][3];
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          pa[par][sl][q] = (q < NP) ? *reinterpret_cast<const uint4 *>(
+                                          Lm + ((size_t)(par * NP + q) * P2p + rt * 16 + r16) * KP32 +
+                                          (sl < nsl ? sl : 0) * 32 + kq * 8)
+                                    : make_uint4(0, 0, 0, 0);
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+      const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl) {
+        if (sl >= nsl) break;
+        slice_mac<MODE>(pa[par][sl], pb[par][sl], Form<MODE>::kScaled ? ea * bsc[par][sl] : 1.0, acc);
+      }
+      if (cok) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int p = rt * 16 + kq * 4 + i;
+          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[i];
+        }
+      }
+    }
+  }
+}
+
+template <int MODE, int NSL>
+__global__ __launch_bounds__(256) void leg_fwd_mixed_wave_kernel(
+    const float *__restrict__ four, double *__restrict__ spec, const uint16_t *__restrict__ L,
+    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
+    int mlen, const double *__restrict__ zero, int N, int P2, int P2p32, int ncol, int nct,
+    FourGeom geo) {
+  constexpr int NP = Form<MODE>::NP;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + 3) >> 2, m, t)) return;
+  const int ct = t * 4 + wave;
+  if (ct >= nct) return;
+  const int M1 = N + 1;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KR = (ke + 15) & ~15, nrt = KR >> 4;
+  const int nsl = P2p32 >> 5;
+  const int col = ct * 16 + r16;
+  const bool cok = col < ncol;
+  const int csafe = cok ? col : 0;
+  const float *__restrict__ fz = reinterpret_cast<const float *>(zero);
+  const uint16_t *__restrict__ Lm = L + loff[m];
+  uint4 pb[2][NSL][3];
+  double bsc[2][NSL];
+#pragma unroll
+  for (int q = 0; q < 2; ++q)
+#pragma unroll
+    for (int sl = 0; sl < NSL; ++sl) {
+      double v[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const int p = sl * 32 + kq * 8 + i;
+        const bool ok = sl < nsl && cok && p < P2;
+        v[i] = (double)*(ok ? four + geo.at(p, m, q, csafe) : fz);
+      }
+      const int e = col_exponent<MODE>(v);
+      bsc[q][sl] = ldexp(1.0, -e);
+      split8<MODE>(v, e, pb[q][sl]);
+    }
+  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + col;
+#pragma unroll
+  for (int par = 0; par < 2; ++par) {  // unrolled: pb[par] stays in registers
+    const int kc = par ? ko : ke;
+    const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
+    for (int kt = 0; kt < nrt; ++kt) {
+      uint4 pa[NSL][3];
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          pa[sl][q] = (q < NP) ? *reinterpret_cast<const uint4 *>(
+                                     Lm + ((size_t)(par * NP + q) * KR + kt * 16 + r16) * P2p32 +
+                                     (sl < nsl ? sl : 0) * 32 + kq * 8)
+                               : make_uint4(0, 0, 0, 0);
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int sl = 0; sl < NSL; ++sl) {
+        if (sl >= nsl) break;
+        slice_mac<MODE>(pa[sl], pb[par][sl], Form<MODE>::kScaled ? ea * bsc[par][sl] : 1.0, acc);
+      }
+      if (cok) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = kt * 16 + kq * 4 + i;
+          if (k < kc) sm[(size_t)(2 * k + par) * ncol] = acc[i];
+        }
+      }
+    }
+  }
+}
+
 // Plan time: split and transpose one (m, parity) block of an FP64 table into
 // the 16-bit piece layout.  For SCALED_FP16 the block's largest magnitude sets
 // the power of two e (stored value = x 2^e), written to lexp[2m + par].
@@ -419,8 +591,16 @@ int build_mixed_mode(Plan &p) {
 }
 
 template <int MODE, int NSL>
-int inv_nsl(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  if (shape == SHT_SHAPE_WAVE) {
+    launch_kernel(leg_inv_mixed_wave_kernel<MODE, NSL>, xcd_m_blocks(p, 0, (nct + 3) / 4), 256, 0, s,
+                  spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv,
+                  p.xcd_m[0], p.xcd_mlen[0], p.zero, p.N, p.P2, p.P2p, ncol, nct,
+                  four_geom(p.N, p.im, nf));
+    SHT_HIP(hipGetLastError());
+    return SHT_OK;
+  }
   launch_kernel(leg_inv_mixed_kernel<MODE, NSL>, xcd_m_blocks(p, 0, nct), 256, 0, s,
       spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
       p.xcd_mlen[0], p.N, p.P2, p.P2p,
@@ -430,19 +610,27 @@ int inv_nsl(const Plan &p, const double *spec, float *four, int nf, hipStream_t 
 }
 
 template <int MODE>
-int inv_mode(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+int inv_mode(const Plan &p, int shape, const double *spec, float *four, int nf, hipStream_t s) {
   switch (roundup(ceil_half(p.N + 1), 32) / 32) {
-    case 1: return inv_nsl<MODE, 1>(p, spec, four, nf, s);
-    case 2: return inv_nsl<MODE, 2>(p, spec, four, nf, s);
-    case 3: return inv_nsl<MODE, 3>(p, spec, four, nf, s);
-    case 4: return inv_nsl<MODE, 4>(p, spec, four, nf, s);
+    case 1: return inv_nsl<MODE, 1>(p, shape, spec, four, nf, s);
+    case 2: return inv_nsl<MODE, 2>(p, shape, spec, four, nf, s);
+    case 3: return inv_nsl<MODE, 3>(p, shape, spec, four, nf, s);
+    case 4: return inv_nsl<MODE, 4>(p, shape, spec, four, nf, s);
   }
   return fail(SHT_ERR_UNSUPPORTED, "mixed inverse: nmax above 255");
 }
 
 template <int MODE, int NSL>
-int fwd_nsl(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+int fwd_nsl(const Plan &p, int shape, const float *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  if (shape == SHT_SHAPE_WAVE) {
+    launch_kernel(leg_fwd_mixed_wave_kernel<MODE, NSL>, xcd_m_blocks(p, 1, (nct + 3) / 4), 256, 0, s,
+                  four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd,
+                  p.xcd_m[1], p.xcd_mlen[1], p.zero, p.N, p.P2, roundup(p.P2p, 32), ncol, nct,
+                  four_geom(p.N, p.im, nf));
+    SHT_HIP(hipGetLastError());
+    return SHT_OK;
+  }
   launch_kernel(leg_fwd_mixed_kernel<MODE, NSL>, xcd_m_blocks(p, 1, nct), 256, 0, s,
       four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
       p.xcd_mlen[1], p.N, p.P2,
@@ -452,16 +640,16 @@ int fwd_nsl(const Plan &p, const float *four, double *spec, int nf, hipStream_t 
 }
 
 template <int MODE>
-int fwd_mode(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+int fwd_mode(const Plan &p, int shape, const float *four, double *spec, int nf, hipStream_t s) {
   switch (roundup(p.P2p, 32) / 32) {
-    case 1: return fwd_nsl<MODE, 1>(p, four, spec, nf, s);
-    case 2: return fwd_nsl<MODE, 2>(p, four, spec, nf, s);
-    case 3: return fwd_nsl<MODE, 3>(p, four, spec, nf, s);
-    case 4: return fwd_nsl<MODE, 4>(p, four, spec, nf, s);
-    case 5: return fwd_nsl<MODE, 5>(p, four, spec, nf, s);
-    case 6: return fwd_nsl<MODE, 6>(p, four, spec, nf, s);
-    case 7: return fwd_nsl<MODE, 7>(p, four, spec, nf, s);
-    case 8: return fwd_nsl<MODE, 8>(p, four, spec, nf, s);
+    case 1: return fwd_nsl<MODE, 1>(p, shape, four, spec, nf, s);
+    case 2: return fwd_nsl<MODE, 2>(p, shape, four, spec, nf, s);
+    case 3: return fwd_nsl<MODE, 3>(p, shape, four, spec, nf, s);
+    case 4: return fwd_nsl<MODE, 4>(p, shape, four, spec, nf, s);
+    case 5: return fwd_nsl<MODE, 5>(p, shape, four, spec, nf, s);
+    case 6: return fwd_nsl<MODE, 6>(p, shape, four, spec, nf, s);
+    case 7: return fwd_nsl<MODE, 7>(p, shape, four, spec, nf, s);
+    case 8: return fwd_nsl<MODE, 8>(p, shape, four, spec, nf, s);
   }
   return fail(SHT_ERR_UNSUPPORTED, "mixed forward: jm above 512");
 }
@@ -477,22 +665,32 @@ int build_mixed_tables(Plan &p) {
   return fail(SHT_ERR_INVALID, "build_mixed_tables: not a mixed-precision plan");
 }
 
-int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+int launch_leg_inv_mixed_shape(const Plan &p, int shape, const double *spec, float *four, int nf,
+                               hipStream_t s) {
   switch (p.prec) {
-    case SHT_PREC_SPLIT_BF16: return inv_mode<SHT_PREC_SPLIT_BF16>(p, spec, four, nf, s);
-    case SHT_PREC_SCALED_FP16: return inv_mode<SHT_PREC_SCALED_FP16>(p, spec, four, nf, s);
-    case SHT_PREC_SPLIT_BF16X6: return inv_mode<SHT_PREC_SPLIT_BF16X6>(p, spec, four, nf, s);
+    case SHT_PREC_SPLIT_BF16: return inv_mode<SHT_PREC_SPLIT_BF16>(p, shape, spec, four, nf, s);
+    case SHT_PREC_SCALED_FP16: return inv_mode<SHT_PREC_SCALED_FP16>(p, shape, spec, four, nf, s);
+    case SHT_PREC_SPLIT_BF16X6: return inv_mode<SHT_PREC_SPLIT_BF16X6>(p, shape, spec, four, nf, s);
   }
   return fail(SHT_ERR_INVALID, "launch_leg_inv_mixed: not a mixed-precision plan");
 }
 
-int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+int launch_leg_fwd_mixed_shape(const Plan &p, int shape, const float *four, double *spec, int nf,
+                               hipStream_t s) {
   switch (p.prec) {
-    case SHT_PREC_SPLIT_BF16: return fwd_mode<SHT_PREC_SPLIT_BF16>(p, four, spec, nf, s);
-    case SHT_PREC_SCALED_FP16: return fwd_mode<SHT_PREC_SCALED_FP16>(p, four, spec, nf, s);
-    case SHT_PREC_SPLIT_BF16X6: return fwd_mode<SHT_PREC_SPLIT_BF16X6>(p, four, spec, nf, s);
+    case SHT_PREC_SPLIT_BF16: return fwd_mode<SHT_PREC_SPLIT_BF16>(p, shape, four, spec, nf, s);
+    case SHT_PREC_SCALED_FP16: return fwd_mode<SHT_PREC_SCALED_FP16>(p, shape, four, spec, nf, s);
+    case SHT_PREC_SPLIT_BF16X6: return fwd_mode<SHT_PREC_SPLIT_BF16X6>(p, shape, four, spec, nf, s);
   }
   return fail(SHT_ERR_INVALID, "launch_leg_fwd_mixed: not a mixed-precision plan");
+}
+
+int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
+  return launch_leg_inv_mixed_shape(p, resolve_shape(p, nf, 0), spec, four, nf, s);
+}
+
+int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {
+  return launch_leg_fwd_mixed_shape(p, resolve_shape(p, nf, 1), four, spec, nf, s);
 }
 
 }  // namespace sht

@@ -16,7 +16,9 @@
 namespace sht {
 namespace {
 
-int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_NSHAPE : 1; }
+// FP64 plans: every shape; mixed-precision plans: KSPLIT (shared LDS data
+// tile) and WAVE (data tile per wave in registers).
+int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_NSHAPE : 2; }
 
 bool shape_ok(const Plan &p, int s) { return s == SHT_SHAPE_AUTO || (s >= 0 && s < nshapes(p)); }
 
@@ -162,14 +164,22 @@ int sht_plan_autotune(sht_plan *h, int nf, int reps, const char *cache_path, dou
       rc = fail(SHT_ERR_NOMEM, "autotune: spectral buffer allocation failed");
     if (!rc && hipMemsetAsync(spec, 0, sizeof(double) * nspec, s) != hipSuccess)
       rc = fail(SHT_ERR_DEVICE, "autotune: hipMemsetAsync");
+    // three interleaved rounds; the lowest per-round mean wins, so a drift in
+    // clock or background load hits every shape alike
+    for (int round = 0; round < 3 && !rc; ++round)
     for (int dir = 0; dir < 2 && !rc; ++dir)
       for (int shape = 0; shape < nshapes(p) && !rc; ++shape) {
         double us = 0.0;
         rc = time_launches(s, reps, [&] {
+          if (p.prec != SHT_PREC_FP64) {
+            float *f4 = reinterpret_cast<float *>(p.four);
+            return dir == 0 ? launch_leg_inv_mixed_shape(p, shape, spec, f4, nf, s)
+                            : launch_leg_fwd_mixed_shape(p, shape, f4, spec, nf, s);
+          }
           return dir == 0 ? launch_leg_inv_f64_shape(p, shape, spec, p.four, p.four2, true, false, nf, s)
                           : launch_leg_fwd_f64_shape(p, shape, p.four, spec, nf, s);
         }, us);
-        if (!rc && (shape == 0 || us < tbest[dir])) {
+        if (!rc && ((round == 0 && shape == 0) || us < tbest[dir])) {
           tbest[dir] = us;
           best[dir] = shape;
         }

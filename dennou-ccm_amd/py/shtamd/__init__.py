@@ -95,6 +95,7 @@ SIGNATURES = {
                                ctypes.POINTER(_ci)]),
     "sht_batch_run": (_ci, [_vp, _ci, _vp]),
     "sht_batch_run_eager": (_ci, [_vp, _ci, _vp]),
+    "sht_batch_autotune": (_ci, [_vp, _ci, ctypes.c_char_p]),
     "sht_server_create": (_ci, [ctypes.POINTER(_ServerDesc), ctypes.POINTER(_vp)]),
     "sht_server_destroy": (None, [_vp]),
     "sht_server_submit": (_ci, [_vp, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_longlong)]),
@@ -311,6 +312,10 @@ class Batch:
 
     def run(self, direction: int = DIR_ROUNDTRIP, stream=None):
         check(lib().sht_batch_run(self._h, direction, _stream_handle(stream)))
+
+    def autotune(self, reps: int = 10, cache_path: str | None = None):
+        """Tune every group's Legendre shapes at its field count; recapture on the next run."""
+        check(lib().sht_batch_autotune(self._h, reps, cache_path.encode() if cache_path else None))
 
     def run_eager(self, direction: int = DIR_ROUNDTRIP, stream=None):
         check(lib().sht_batch_run_eager(self._h, direction, _stream_handle(stream)))

@@ -174,6 +174,11 @@ int sht_batch_member(const sht_batch *batch, int j, double **spec_in, double **g
 int sht_batch_run(sht_batch *batch, int direction, void *stream);
 /* The same work launched kernel by kernel (no graph), for comparison. */
 int sht_batch_run_eager(sht_batch *batch, int direction, void *stream);
+/* Autotunes the Legendre kernel shapes of every group's plan at the group's
+ * field count (sht_plan_autotune; cache_path may be NULL), then drops the
+ * captured graphs so the next sht_batch_run captures the tuned kernels.
+ * Synchronizes the device; call it while the batch is idle. */
+int sht_batch_autotune(sht_batch *batch, int reps, const char *cache_path);
 
 /* ---- request server: continuous batching of transform requests whose data
  *      are resident in HBM.  No SPML counterpart (SPML is a library called by

@@ -101,3 +101,22 @@ def test_batch_inverse_forward_directions():
         _, s = read_member(b, j)
         assert rel(s, specs[j]) < 1e-12
     b.close()
+
+
+def test_batch_autotune_keeps_results():
+    """Autotuning a batch re-captures its graph with the tuned shapes; results
+    stay within the FP64 bar and the graph still equals the eager run."""
+    b, specs = make_batch(shtamd.PREC_FP64)
+    b.autotune(reps=3)
+    b.run(shtamd.DIR_ROUNDTRIP)
+    torch.cuda.synchronize()
+    graph_out = [read_member(b, j) for j in range(len(MEMBERS))]
+    b.run_eager(shtamd.DIR_ROUNDTRIP)
+    torch.cuda.synchronize()
+    for j, (N, nf) in enumerate(MEMBERS):
+        im, jm = GRIDS[N]
+        g, s = read_member(b, j)
+        assert np.array_equal(g, graph_out[j][0]) and np.array_equal(s, graph_out[j][1]), j
+        assert rel(g, oracle.Plan(N, im, jm).spec_to_grid(specs[j])) < 1e-12, j
+        assert rel(s, specs[j]) < 1e-12, j
+    b.close()

@@ -124,6 +124,23 @@ def test_shape_results_do_not_depend_on_batch():
     p.close()
 
 
+@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (42, 128, 64, 9), (170, 512, 256, 13)])
+def test_wave_and_tile_inverse_bitwise_equal(N, im, jm, nf):
+    """WAVE and TILE run the same MFMA sequence per output tile (steps in order,
+    padded steps adding exact zeros), so the default inverse shape may switch
+    between them with the field count without changing any result."""
+    say("wave vs tile", N, nf)
+    spec = synth.spectra(N, nf, cfg=15)
+    p = shtamd.Plan(N, im, jm, nf, flags=shtamd.FLAG_GRAD)
+    out = []
+    for sh in (shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE):
+        p.set_shapes(0, sh, shtamd.SHAPE_KSPLIT)
+        out.append(run_inverse(p, spec, nf, grad=True))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+    p.close()
+
+
 def test_autotune_and_cache(tmp_path):
     N, im, jm, nf = 85, 256, 128, 97
     cache = str(tmp_path / "shapes.json")
@@ -161,8 +178,36 @@ def test_shape_argument_checks():
     p.close()
     m = shtamd.Plan(21, 64, 32, 8, precision=shtamd.PREC_SPLIT_BF16X6)
     with pytest.raises(shtamd.ShtError):
-        m.set_shapes(0, shtamd.SHAPE_WAVE, shtamd.SHAPE_WAVE)
-    m.set_shapes(0, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT)
+        m.set_shapes(0, shtamd.SHAPE_TILE, shtamd.SHAPE_TILE)
+    m.set_shapes(0, shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
+    assert m.get_shapes(5) == (shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     (inv, fwd), _ = m.autotune(8, reps=2)
-    assert (inv, fwd) == (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT)
+    assert inv in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE) and fwd in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE)
     m.close()
+
+
+MIXED_TOL = {shtamd.PREC_SPLIT_BF16: 5e-5, shtamd.PREC_SCALED_FP16: 1e-6, shtamd.PREC_SPLIT_BF16X6: 1e-6}
+
+
+@pytest.mark.parametrize("prec", sorted(MIXED_TOL))
+@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (21, 64, 32, 9), (170, 512, 256, 7), (31, 64, 48, 17)])
+def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
+    """The two mixed-precision shapes split the same operands into the same
+    pieces and sum the same products in the same order: bitwise equal results,
+    within the form's bound of the oracle."""
+    say("mixed shapes", prec, N, nf)
+    spec = synth.spectra(N, nf, cfg=14)
+    ref = oracle.Plan(N, im, jm)
+    g_ref = ref.spec_to_grid(spec)
+    p = shtamd.Plan(N, im, jm, nf, precision=prec)
+    out = []
+    for sh in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE):
+        p.set_shapes(0, sh, sh)
+        g = run_inverse(p, spec, nf)[0]
+        s = run_forward(p, g_ref, nf)
+        assert rel(g, g_ref) < MIXED_TOL[prec]
+        assert rel(s, spec) < 2 * MIXED_TOL[prec]
+        out.append((g, s))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    p.close()
