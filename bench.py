@@ -71,7 +71,7 @@ def rel(a, b):
     return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
 
 
-def run_gpu(precision: int, steps: int, warmup: int):
+def run_gpu(precision: int, steps: int, warmup: int, autotune: bool = True):
     import torch
 
     import shtamd
@@ -80,7 +80,10 @@ def run_gpu(precision: int, steps: int, warmup: int):
     dev = torch.device("cuda:0")
     spec_c = synth.state_spectra(N, NLEV)
     plan = shtamd.Plan(N, IM, JM, NF, precision=precision)
-    shapes, _ = plan.autotune(NF, reps=20)  # Legendre kernel shapes for this batch
+    if autotune:
+        shapes, _ = plan.autotune(NF, reps=20)  # Legendre kernel shapes for this batch
+    else:
+        shapes = plan.get_shapes(NF)  # the built-in choice
     s_in = torch.from_numpy(shtamd.spec_to_real(spec_c)).to(dev)
     grid = torch.empty((NF, JM, IM), dtype=torch.float64, device=dev)
     s_out = torch.empty_like(s_in)
@@ -322,6 +325,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests in the 500 req/s serving run of the headline line (0 = skip)")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--shapes", default="auto", choices=["auto", "default"],
+                    help="Legendre kernel shapes: autotuned per form (auto) or the built-in choice "
+                         "(default; what the rocprof summaries use, without the tuner's trial launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -346,7 +352,7 @@ def main():
     ref = oracle.Plan(N, IM, JM)
     results = {}
     for form in forms:
-        res = run_gpu(PRECISIONS[form], args.steps, args.warmup)
+        res = run_gpu(PRECISIONS[form], args.steps, args.warmup, autotune=args.shapes == "auto")
         g_ref = ref.spec_to_grid(res["spec_in"])
         res["err"] = {"grid_rel_l2": rel(res["grid"], g_ref),
                       "spec_rel_l2": rel(res["spec"], ref.grid_to_spec(res["grid"])),

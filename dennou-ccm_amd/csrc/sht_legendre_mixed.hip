@@ -94,39 +94,75 @@ __device__ __forceinline__ f4 mfma16(const uint4 &a, const uint4 &b, f4 c) {
                                                    __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-// One K slice: adds the slice's contribution, unscaled, to acc in FP64.
+// Products of one K slice, in two independent FP32 chains (the small cross
+// terms, then the large ones) so that consecutive MFMAs do not wait on each
+// other; the chains meet in FP64.
 template <int MODE>
-__device__ __forceinline__ void slice_mac(const uint4 (&pa)[3], const uint4 (&pb)[3], double scale,
-                                          d4 &acc) {
+struct SliceChains {
+  f4 small, big;
+};
+
+template <int MODE>
+__device__ __forceinline__ SliceChains<MODE> slice_products(const uint4 (&pa)[3], const uint4 (&pb)[3]) {
   const f4 z = {0.f, 0.f, 0.f, 0.f};
+  SliceChains<MODE> c;
   if constexpr (MODE == SHT_PREC_SCALED_FP16) {
-    f4 c1 = mfma16<MODE>(pa[0], pb[1], z);
-    c1 = mfma16<MODE>(pa[1], pb[0], c1);
-    f4 c0 = mfma16<MODE>(pa[0], pb[0], z);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] += ((double)c0[i] + (double)c1[i] * (1.0 / 2048.0)) * scale;
+    // small: the lo pieces (stored 2^11 larger), big: hi.hi
+    c.small = mfma16<MODE>(pa[0], pb[1], z);
+    c.big = mfma16<MODE>(pa[0], pb[0], z);
+    c.small = mfma16<MODE>(pa[1], pb[0], c.small);
   } else if constexpr (MODE == SHT_PREC_SPLIT_BF16X6) {
-    f4 c = mfma16<MODE>(pa[1], pb[1], z);
-    c = mfma16<MODE>(pa[0], pb[2], c);
-    c = mfma16<MODE>(pa[2], pb[0], c);
-    c = mfma16<MODE>(pa[0], pb[1], c);
-    c = mfma16<MODE>(pa[1], pb[0], c);
-    c = mfma16<MODE>(pa[0], pb[0], c);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] += (double)c[i];
+    c.small = mfma16<MODE>(pa[1], pb[1], z);
+    c.big = mfma16<MODE>(pa[0], pb[1], z);
+    c.small = mfma16<MODE>(pa[0], pb[2], c.small);
+    c.big = mfma16<MODE>(pa[1], pb[0], c.big);
+    c.small = mfma16<MODE>(pa[2], pb[0], c.small);
+    c.big = mfma16<MODE>(pa[0], pb[0], c.big);
   } else {
-    f4 c = mfma16<MODE>(pa[0], pb[1], z);
-    c = mfma16<MODE>(pa[1], pb[0], c);
-    c = mfma16<MODE>(pa[0], pb[0], c);
+    c.small = mfma16<MODE>(pa[0], pb[1], z);
+    c.big = mfma16<MODE>(pa[0], pb[0], z);
+    c.small = mfma16<MODE>(pa[1], pb[0], c.small);
+  }
+  return c;
+}
+
+// Adds one slice's contribution, unscaled, to acc in FP64.
+template <int MODE>
+__device__ __forceinline__ void slice_add(const SliceChains<MODE> &c, double scale, d4 &acc) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) acc[i] += (double)c[i];
+  for (int i = 0; i < 4; ++i) {
+    if constexpr (MODE == SHT_PREC_SCALED_FP16)
+      acc[i] += ((double)c.big[i] + (double)c.small[i] * (1.0 / 2048.0)) * scale;
+    else
+      acc[i] += (double)c.small[i] + (double)c.big[i];
   }
 }
 
+template <int MODE>
+__device__ __forceinline__ void slice_mac(const uint4 (&pa)[3], const uint4 (&pb)[3], double scale,
+                                          d4 &acc) {
+  slice_add<MODE>(slice_products<MODE>(pa, pb), scale, acc);
+}
+
+// Two slices (the two parities of one row tile), products issued together so
+// four independent chains alternate on the matrix core.
+template <int MODE>
+__device__ __forceinline__ void slice_mac2(const uint4 (&pa0)[3], const uint4 (&pb0)[3], double sc0,
+                                           d4 &acc0, const uint4 (&pa1)[3], const uint4 (&pb1)[3],
+                                           double sc1, d4 &acc1) {
+  const SliceChains<MODE> c0 = slice_products<MODE>(pa0, pb0);
+  const SliceChains<MODE> c1 = slice_products<MODE>(pa1, pb1);
+  slice_add<MODE>(c0, sc0, acc0);
+  slice_add<MODE>(c1, sc1, acc1);
+}
+
 // Data tile of one workgroup: for each parity and K slice, the 16 columns x 32
-// K values split into pieces, laid out [group][piece][col][32] (16-bit) so a
-// lane's 8 K values are one 16-byte LDS read; ebits[group][col] holds the FP16
-// scaling exponent.  group = par * maxsl + slice.
+// K values split into pieces, laid out [group][piece][col][kRowPitch] (16-bit)
+// so a lane's 8 K values are one 16-byte LDS read; ebits[group][col] holds the
+// FP16 scaling exponent.  group = par * maxsl + slice.  A column row is 32
+// values padded to 40 (80 bytes): eight lanes reading the same K octet of
+// consecutive columns then hit eight different 16-byte bank groups.
+constexpr int kRowPitch = 40;
 // Every wave converts whole groups (wave w takes groups w, w+4, ...): lane l
 // reads the 8 values K = 8 (l/16) .. +7 of column l % 16 (load(g, k, c) returns
 // the FP64 value or 0 outside the tile), the FP16 exponent comes from the
@@ -162,7 +198,7 @@ __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups
     }
 #pragma unroll
     for (int q = 0; q < NP; ++q)
-      *reinterpret_cast<uint4 *>(bt + (((size_t)g * NP + q) * 16 + c) * 32 + k8 * 8) =
+      *reinterpret_cast<uint4 *>(bt + (((size_t)g * NP + q) * 16 + c) * kRowPitch + k8 * 8) =
           make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
   }
 }
@@ -172,7 +208,7 @@ __device__ __forceinline__ void load_pb(const uint16_t *bt, int g, int kq, int r
   constexpr int NP = Form<MODE>::NP;
 #pragma unroll
   for (int q = 0; q < 3; ++q)
-    pb[q] = (q < NP) ? *reinterpret_cast<const uint4 *>(bt + (((size_t)g * NP + q) * 16 + r16) * 32 + kq * 8)
+    pb[q] = (q < NP) ? *reinterpret_cast<const uint4 *>(bt + (((size_t)g * NP + q) * 16 + r16) * kRowPitch + kq * 8)
                      : make_uint4(0, 0, 0, 0);
 }
 
@@ -190,7 +226,7 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
     int mlen, int N, int P2, int P2p, int ncol, int nct, FourGeom geo) {
   constexpr int NP = Form<MODE>::NP;
-  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * 32];
+  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * kRowPitch];
   __shared__ int ebits[2 * NSL * 16];
   int m, ct;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
@@ -235,25 +271,27 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
           for (int q = 0; q < 3; ++q)
             pa[par][s][q] = (q < NP) ? a_frag(rt, par, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
     }
+    const double ea0 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m]) : 1.0;
+    const double ea1 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + 1]) : 1.0;
+    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
-      const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < NSL; ++s) {
+      if (s >= nsl) break;
+      uint4 pb0[3], pb1[3];
+      load_pb<MODE>(bt, s, kq, r16, pb0);
+      load_pb<MODE>(bt, NSL + s, kq, r16, pb1);
+      const double sc0 = Form<MODE>::kScaled ? ea0 * ldexp(1.0, -ebits[s * 16 + r16]) : 1.0;
+      const double sc1 = Form<MODE>::kScaled ? ea1 * ldexp(1.0, -ebits[(NSL + s) * 16 + r16]) : 1.0;
+      slice_mac2<MODE>(pa[0][s], pb0, sc0, acc[0], pa[1][s], pb1, sc1, acc[1]);
+    }
+    if (col < ncol) {
 #pragma unroll
-      for (int s = 0; s < NSL; ++s) {
-        if (s >= nsl) break;
-        const int g = par * NSL + s;
-        uint4 pb[3];
-        load_pb<MODE>(bt, g, kq, r16, pb);
-        slice_mac<MODE>(pa[par][s], pb, Form<MODE>::kScaled ? ea * ldexp(1.0, -ebits[g * 16 + r16]) : 1.0, acc);
-      }
-      if (col < ncol) {
+      for (int par = 0; par < 2; ++par)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int p = rt * 16 + kq * 4 + i;
-          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[i];
+          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[par][i];
         }
-      }
     }
   }
 }
@@ -267,7 +305,7 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
     int mlen, int N, int P2, int P2p32, int ncol, int nct, FourGeom geo) {
   constexpr int NP = Form<MODE>::NP;
-  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * 32];
+  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * kRowPitch];
   __shared__ int ebits[2 * NSL * 16];
   int m, ct;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
@@ -413,22 +451,23 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
                                           Lm + ((size_t)(par * NP + q) * P2p + rt * 16 + r16) * KP32 +
                                           (sl < nsl ? sl : 0) * 32 + kq * 8)
                                     : make_uint4(0, 0, 0, 0);
+    const double ea0 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m]) : 1.0;
+    const double ea1 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + 1]) : 1.0;
+    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-    for (int par = 0; par < 2; ++par) {
-      const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
-      d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int sl = 0; sl < NSL; ++sl) {
+      if (sl >= nsl) break;
+      slice_mac2<MODE>(pa[0][sl], pb[0][sl], Form<MODE>::kScaled ? ea0 * bsc[0][sl] : 1.0, acc[0],
+                       pa[1][sl], pb[1][sl], Form<MODE>::kScaled ? ea1 * bsc[1][sl] : 1.0, acc[1]);
+    }
+    if (cok) {
 #pragma unroll
-      for (int sl = 0; sl < NSL; ++sl) {
-        if (sl >= nsl) break;
-        slice_mac<MODE>(pa[par][sl], pb[par][sl], Form<MODE>::kScaled ? ea * bsc[par][sl] : 1.0, acc);
-      }
-      if (cok) {
+      for (int par = 0; par < 2; ++par)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int p = rt * 16 + kq * 4 + i;
-          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[i];
+          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[par][i];
         }
-      }
     }
   }
 }

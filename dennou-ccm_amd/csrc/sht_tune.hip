@@ -156,13 +156,15 @@ int sht_plan_autotune(sht_plan *h, int nf, int reps, const char *cache_path, dou
     SHT_HIP(hipGetDevice(&cur));
     SHT_HIP(hipSetDevice(p.device));
     hipStream_t s = nullptr;
-    double *spec = nullptr;
+    double *spec = nullptr, *grid = nullptr;
     int rc = SHT_OK;
     if (hipStreamCreate(&s) != hipSuccess) rc = fail(SHT_ERR_DEVICE, "autotune: hipStreamCreate");
-    const size_t nspec = (size_t)p.K * 2 * nf;
-    if (!rc && hipMalloc(&spec, sizeof(double) * nspec) != hipSuccess)
-      rc = fail(SHT_ERR_NOMEM, "autotune: spectral buffer allocation failed");
-    if (!rc && hipMemsetAsync(spec, 0, sizeof(double) * nspec, s) != hipSuccess)
+    const size_t nspec = (size_t)p.K * 2 * nf, ngrid = (size_t)nf * p.jm * p.im;
+    if (!rc && (hipMalloc(&spec, sizeof(double) * nspec) != hipSuccess ||
+                hipMalloc(&grid, sizeof(double) * ngrid) != hipSuccess))
+      rc = fail(SHT_ERR_NOMEM, "autotune: buffer allocation failed");
+    if (!rc && (hipMemsetAsync(spec, 0, sizeof(double) * nspec, s) != hipSuccess ||
+                hipMemsetAsync(grid, 0, sizeof(double) * ngrid, s) != hipSuccess))
       rc = fail(SHT_ERR_DEVICE, "autotune: hipMemsetAsync");
     // three interleaved rounds; the lowest per-round mean wins, so a drift in
     // clock or background load hits every shape alike
@@ -170,14 +172,22 @@ int sht_plan_autotune(sht_plan *h, int nf, int reps, const char *cache_path, dou
     for (int dir = 0; dir < 2 && !rc; ++dir)
       for (int shape = 0; shape < nshapes(p) && !rc; ++shape) {
         double us = 0.0;
+        // a whole direction (Legendre and FFT) per launch, as a caller runs it:
+        // the shapes differ in how they share the chip with the FFT's traffic
+        const bool f32 = p.prec != SHT_PREC_FP64;
+        float *f4 = reinterpret_cast<float *>(p.four);
         rc = time_launches(s, reps, [&] {
-          if (p.prec != SHT_PREC_FP64) {
-            float *f4 = reinterpret_cast<float *>(p.four);
-            return dir == 0 ? launch_leg_inv_mixed_shape(p, shape, spec, f4, nf, s)
-                            : launch_leg_fwd_mixed_shape(p, shape, f4, spec, nf, s);
+          int r;
+          if (dir == 0) {
+            r = f32 ? launch_leg_inv_mixed_shape(p, shape, spec, f4, nf, s)
+                    : launch_leg_inv_f64_shape(p, shape, spec, p.four, p.four2, true, false, nf, s);
+            if (!r) r = launch_fft_inv(p, p.four, nullptr, f32, grid, nullptr, nullptr, nf, s);
+          } else {
+            r = launch_fft_fwd(p, grid, p.four, f32, nf, s);
+            if (!r) r = f32 ? launch_leg_fwd_mixed_shape(p, shape, f4, spec, nf, s)
+                            : launch_leg_fwd_f64_shape(p, shape, p.four, spec, nf, s);
           }
-          return dir == 0 ? launch_leg_inv_f64_shape(p, shape, spec, p.four, p.four2, true, false, nf, s)
-                          : launch_leg_fwd_f64_shape(p, shape, p.four, spec, nf, s);
+          return r;
         }, us);
         if (!rc && ((round == 0 && shape == 0) || us < tbest[dir])) {
           tbest[dir] = us;
@@ -185,6 +195,7 @@ int sht_plan_autotune(sht_plan *h, int nf, int reps, const char *cache_path, dou
         }
       }
     if (spec) (void)hipFree(spec);
+    if (grid) (void)hipFree(grid);
     if (s) (void)hipStreamDestroy(s);
     (void)hipSetDevice(cur);
     if (rc) return rc;

@@ -258,11 +258,12 @@ int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
 /* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
 int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);
 /* Times every shape of both Legendre kernels at nf fields on the plan's device
- * (reps launches each, on a private stream; synchronizes it) and keeps the
+ * (reps runs of the whole direction, Legendre with FFT, per shape in three
+ * interleaved rounds, on a private stream; synchronizes it) and keeps the
  * fastest for calls with nf fields.  With cache_path not NULL, a JSON file
  * keyed by truncation, grid, nf and precision is consulted first and updated
  * after timing.  best_us (2 doubles, may be NULL) receives the winners' mean
- * durations in microseconds (0 when taken from the cache).  It uses the plan's
+ * direction times in microseconds (0 when taken from the cache).  It uses the plan's
  * scratch: call it while no other call on the plan is in flight. */
 int sht_plan_autotune(sht_plan *plan, int nf, int reps, const char *cache_path, double *best_us);
 
