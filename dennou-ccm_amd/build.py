@@ -26,7 +26,7 @@ FLAGS = [
 ]
 SOURCES = ["sht_capi.hip", "sht_tables.hip", "sht_fft.hip", "sht_legendre.hip",
            "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip", "sht_server.hip",
-           "sht_tune.hip"]
+           "sht_tune.hip", "sht_remap.hip"]
 
 
 def _compile(src: str) -> str:

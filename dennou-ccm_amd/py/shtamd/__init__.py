@@ -104,6 +104,10 @@ SIGNATURES = {
     "sht_server_stats": (_ci, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
     "sht_plan_set_timing": (_ci, [_vp, _ci]),
     "sht_plan_stage_times": (_ci, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "sht_remap_create": (_ci, [ctypes.POINTER(_ci), ctypes.POINTER(_ci), _dp, _ci, _ci, _ci,
+                               ctypes.POINTER(_vp)]),
+    "sht_remap_destroy": (None, [_vp]),
+    "sht_remap_apply_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_plan_set_shapes": (_ci, [_vp, _ci, _ci, _ci]),
     "sht_plan_get_shapes": (_ci, [_vp, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_plan_autotune": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _dp]),
@@ -319,6 +323,35 @@ class Batch:
 
     def run_eager(self, direction: int = DIR_ROUNDTRIP, stream=None):
         check(lib().sht_batch_run_eager(self._h, direction, _stream_handle(stream)))
+
+
+class Remap:
+    """Coupler remap (sht_remap_*): recv[d, r] = sum over triplets with recv
+    index r of send[d, s] * coef, in triplet order; 0-based indices."""
+
+    def __init__(self, send_index, recv_index, coef, nsend: int, nrecv: int):
+        si = np.ascontiguousarray(send_index, dtype=np.int32)
+        ri = np.ascontiguousarray(recv_index, dtype=np.int32)
+        cf = np.ascontiguousarray(coef, dtype=np.float64)
+        h = ctypes.c_void_p()
+        check(lib().sht_remap_create(si.ctypes.data_as(ctypes.POINTER(_ci)), ri.ctypes.data_as(ctypes.POINTER(_ci)),
+                                     _host(cf), len(cf), nsend, nrecv, ctypes.byref(h)))
+        self._h = h
+        self.nsend, self.nrecv = nsend, nrecv
+
+    def apply(self, send, recv, ndata: int, stream=None):
+        check(lib().sht_remap_apply_dev(self._h, _ptr(send), _ptr(recv), ndata, _stream_handle(stream)))
+
+    def close(self):
+        if self._h:
+            lib().sht_remap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

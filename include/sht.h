@@ -221,6 +221,21 @@ int sht_server_drain(sht_server *server);
 int sht_server_stats(const sht_server *server, long long *requests, long long *fields,
                      long long *batches, long long *refused);
 
+/* ---- coupler remap: the sparse interpolation Jcup applies to exchanged
+ *      fields (common/interpolation_data_latlon_mod.f90:274-306,
+ *      interpolate_data_latlon):  recv(:, d) = 0;
+ *      recv(r_i, d) += send(s_i, d) * coef_i for every triplet i, in triplet
+ *      order.  Indices are 0-based here (the reference's are 1-based).  The
+ *      GPU sums each receiving point's triplets in their original order with
+ *      unfused multiply and add, so results are bitwise those of the loop. */
+typedef struct sht_remap sht_remap;
+int sht_remap_create(const int *send_index, const int *recv_index, const double *coef, int nnz,
+                     int nsend, int nrecv, sht_remap **remap);
+void sht_remap_destroy(sht_remap *remap);
+/* send [ndata][nsend], recv [ndata][nrecv] (Fortran send_data(:, d)), device. */
+int sht_remap_apply_dev(const sht_remap *remap, const double *send, double *recv, int ndata,
+                        void *stream);
+
 /* ---- stage timers: HIP events around every kernel launch of a plan, summed
  *      per stage -- the counterpart of DCPAM's ProfUtil_RapStart/RapEnd
  *      regions around the dynamical core (atm/dcpam_main_mod.f90:2167, :2203).

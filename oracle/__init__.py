@@ -45,6 +45,8 @@ def lib():
         L.orc_avr_lonlat.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
         L.orc_grid_to_div.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int]
         L.orc_spec_lapla.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_remap.argtypes = [_ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -150,6 +152,20 @@ def spec_lapla(spec, nmax: int, power: int):
     out = np.empty_like(spec)
     lib().orc_spec_lapla(nmax, spec.shape[1], _p(spec.view(np.float64).reshape(-1)),
                          _p(out.view(np.float64).reshape(-1)), power)
+    return out
+
+
+def remap(send_index, recv_index, coef, nrecv: int, send):
+    """The coupler remap loop; send [ndata][nsend] -> recv [ndata][nrecv]."""
+    si = np.ascontiguousarray(send_index, dtype=np.int32)
+    ri = np.ascontiguousarray(recv_index, dtype=np.int32)
+    cf = np.ascontiguousarray(coef, dtype=np.float64)
+    send = np.ascontiguousarray(send, dtype=np.float64)
+    ndata, nsend = send.shape
+    out = np.empty((ndata, nrecv))
+    ip = ctypes.POINTER(ctypes.c_int)
+    lib().orc_remap(si.ctypes.data_as(ip), ri.ctypes.data_as(ip), _p(cf), len(cf), nsend, nrecv, ndata,
+                    _p(send), _p(out))
     return out
 
 

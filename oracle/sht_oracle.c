@@ -574,6 +574,20 @@ void orc_spec_lapla(int nmax, int nf, const double *in, double *out, int power) 
     }
 }
 
+/* Coupler remap, the loop of interpolate_data_latlon
+ * (common/interpolation_data_latlon_mod.f90:274-306) with 0-based indices:
+ * recv(:, d) = 0; then for each d and each triplet i in order,
+ * recv(r_i, d) = recv(r_i, d) + send(s_i, d) * coef_i. */
+void orc_remap(const int *send_index, const int *recv_index, const double *coef, int nnz,
+               int nsend, int nrecv, int ndata, const double *send, double *recv) {
+  for (long t = 0; t < (long)nrecv * ndata; ++t) recv[t] = 0.0;
+  for (int d = 0; d < ndata; ++d)
+    for (int i = 0; i < nnz; ++i) {
+      double *r = recv + (size_t)d * nrecv + recv_index[i];
+      *r = *r + send[(size_t)d * nsend + send_index[i]] * coef[i];
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 
 void orc_spml_pack(int nmax, int nf, const double *spec, double *wa) {

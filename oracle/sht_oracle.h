@@ -73,6 +73,11 @@ void orc_grid_to_div(const orc_plan *p, int nf, const double *a, const double *b
 /* out = (-n(n+1))^power in on canonical spectra (negative powers: n = 0 -> 0) */
 void orc_spec_lapla(int nmax, int nf, const double *in, double *out, int power);
 
+/* Coupler remap (interpolate_data_latlon), 0-based triplets, fields
+ * contiguous per point: send[ndata][nsend] -> recv[ndata][nrecv]. */
+void orc_remap(const int *send_index, const int *recv_index, const double *coef, int nnz,
+               int nsend, int nrecv, int ndata, const double *send, double *recv);
+
 /* Canonical complex <-> real (N+1)^2 packing wa[f][l], l = n(n+1)+m,
  * m = -n..n; wa(n,0) = Re s, wa(n,m>0) = sqrt2 Re s, wa(n,-m) = -sqrt2 Im s
  * (orthonormal real harmonics with unit mean square). */

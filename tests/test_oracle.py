@@ -439,3 +439,39 @@ def test_laplacian_powers():
     want[0] = 0.0
     assert rel(inv, want) < 1e-15
     assert np.all(oracle.spec_lapla(spec, N, -2)[0] == 0.0)
+
+
+# ---- coupler remap (interpolate_data_latlon, common/interpolation_data_latlon_mod.f90:274-306)
+
+def _remap_case(seed=0, nsend=300, nrecv=200, nnz=1500):
+    rng = np.random.default_rng(seed)
+    si = rng.integers(0, nsend, nnz)
+    ri = rng.integers(0, nrecv, nnz)
+    cf = rng.standard_normal(nnz)
+    return si, ri, cf
+
+
+def test_remap_matches_brute_force_loop():
+    si, ri, cf = _remap_case(nsend=17, nrecv=11, nnz=60)
+    send = np.random.default_rng(1).standard_normal((3, 17))
+    want = np.zeros((3, 11))
+    for d in range(3):
+        for i in range(len(cf)):
+            want[d, ri[i]] = want[d, ri[i]] + send[d, si[i]] * cf[i]
+    assert np.array_equal(oracle.remap(si, ri, cf, 11, send), want)
+
+
+def test_remap_vs_scipy_sparse_and_partition_of_unity():
+    import scipy.sparse as sp
+    si, ri, cf = _remap_case()
+    send = np.random.default_rng(2).standard_normal((4, 300))
+    A = sp.coo_matrix((cf, (ri, si)), shape=(200, 300)).tocsr()
+    got = oracle.remap(si, ri, cf, 200, send)
+    assert np.abs(got - (A @ send.T).T).max() < 1e-13
+    # weights normalised per receiving point reproduce a constant field
+    rows = np.bincount(ri, weights=np.abs(cf), minlength=200)
+    w = np.abs(cf) / rows[ri]
+    const = oracle.remap(si, ri, w, 200, np.full((1, 300), 3.5))
+    hit = np.bincount(ri, minlength=200) > 0
+    assert np.allclose(const[0, hit], 3.5, rtol=0, atol=1e-14)
+    assert np.all(const[0, ~hit] == 0.0)
