@@ -168,39 +168,57 @@ constexpr int kRowPitch = 40;
 // the FP64 value or 0 outside the tile), the FP16 exponent comes from the
 // maximum over the group's 32 values (a 4-lane shuffle reduction), and the
 // lane writes each of its pieces as one 16-byte LDS store -- already the
-// matrix-core B layout.
-template <int MODE, class Load>
+// matrix-core B layout.  A wave issues the loads of NIT groups before it
+// converts any, so one memory latency covers them (groups past ngroups load
+// group 0 again and are not stored).
+template <int MODE, int NIT, class Load>
 __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Load load) {
   constexpr int NP = Form<MODE>::NP;
   const int lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
-  for (int g = wave_id(); g < ngroups; g += 4) {
-    double v[8];
+  for (int g0 = wave_id(); g0 < ngroups; g0 += 4 * NIT) {
+    double v[NIT][8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = load(g, k8 * 8 + i, c);
-    int e = 0;
-    if constexpr (Form<MODE>::kScaled) {
-      double mx = 0.0;
+    for (int it = 0; it < NIT; ++it) {
+      const int g = g0 + 4 * it < ngroups ? g0 + 4 * it : 0;
 #pragma unroll
-      for (int i = 0; i < 8; ++i) mx = fmax(mx, fabs(v[i]));
-      mx = fmax(mx, __shfl_xor(mx, 16));
-      mx = fmax(mx, __shfl_xor(mx, 32));
-      e = mx > 0.0 ? 13 - ilogb(mx) : 0;
-      if (k8 == 0) ebits[g * 16 + c] = e;
-    }
-    uint32_t w[3][4];
-#pragma unroll
-    for (int i = 0; i < 8; i += 2) {
-      uint16_t p0[3], p1[3];
-      split<MODE>(Form<MODE>::kScaled ? ldexp(v[i], e) : v[i], p0);
-      split<MODE>(Form<MODE>::kScaled ? ldexp(v[i + 1], e) : v[i + 1], p1);
-#pragma unroll
-      for (int q = 0; q < 3; ++q) w[q][i >> 1] = p0[q] | ((uint32_t)p1[q] << 16);
+      for (int i = 0; i < 8; ++i) v[it][i] = load(g, k8 * 8 + i, c);
     }
 #pragma unroll
-    for (int q = 0; q < NP; ++q)
-      *reinterpret_cast<uint4 *>(bt + (((size_t)g * NP + q) * 16 + c) * kRowPitch + k8 * 8) =
-          make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
+    for (int it = 0; it < NIT; ++it) {
+      const int g = g0 + 4 * it;
+      if (g >= ngroups) break;
+      int e = 0;
+      if constexpr (Form<MODE>::kScaled) {
+        double mx = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) mx = fmax(mx, fabs(v[it][i]));
+        mx = fmax(mx, __shfl_xor(mx, 16));
+        mx = fmax(mx, __shfl_xor(mx, 32));
+        e = mx > 0.0 ? 13 - ilogb(mx) : 0;
+        if (k8 == 0) ebits[g * 16 + c] = e;
+      }
+      uint32_t w[3][4];
+#pragma unroll
+      for (int i = 0; i < 8; i += 2) {
+        uint16_t p0[3], p1[3];
+        split<MODE>(Form<MODE>::kScaled ? ldexp(v[it][i], e) : v[it][i], p0);
+        split<MODE>(Form<MODE>::kScaled ? ldexp(v[it][i + 1], e) : v[it][i + 1], p1);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) w[q][i >> 1] = p0[q] | ((uint32_t)p1[q] << 16);
+      }
+#pragma unroll
+      for (int q = 0; q < NP; ++q)
+        *reinterpret_cast<uint4 *>(bt + (((size_t)g * NP + q) * 16 + c) * kRowPitch + k8 * 8) =
+            make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
+    }
   }
+}
+
+// Groups a wave stages per memory round trip: its share of NGROUPS, at most
+// two, so the staged values (16 registers per group) stay few.
+template <int NGROUPS>
+constexpr int stage_iters() {
+  return (NGROUPS + 3) / 4 < 2 ? (NGROUPS + 3) / 4 : 2;
 }
 
 template <int MODE>
@@ -217,25 +235,28 @@ __device__ __forceinline__ void load_pb(const uint16_t *bt, int g, int kq, int r
 // LDS are sized by it, so small truncations keep a high occupancy.
 
 // Inverse: spec (FP64) -> four (FP32 even/odd Legendre sums).
-// Workgroup = one (m, 16-column tile).  Each wave first issues the table
+// Workgroup = one (m, CT 16-column tiles).  Each wave first issues the table
 // fragments of its first 16-pair row tile, then all 256 threads split the
-// spectral tile into LDS; wave w then takes row tiles w, w+4, ...
-template <int MODE, int NSL>
+// spectral tiles into LDS; wave w then takes row tiles w, w+4, ... and runs
+// each against every column tile with the same table fragments.  CT = 2
+// halves the workgroups, and with them the waves that wait on memory at
+// once, and the table fragments fetched per column.
+template <int MODE, int NSL, int CT>
 __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
     int mlen, int N, int P2, int P2p, int ncol, int nct, FourGeom geo) {
-  constexpr int NP = Form<MODE>::NP;
-  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * kRowPitch];
-  __shared__ int ebits[2 * NSL * 16];
-  int m, ct;
-  if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
+  constexpr int NP = Form<MODE>::NP, NG = 2 * NSL;  // LDS groups per column tile
+  __shared__ __align__(16) uint16_t bt[CT * NG * NP * 16 * kRowPitch];
+  __shared__ int ebits[CT * NG * 16];
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
+  const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
   const int M1 = N + 1;
   const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
-  const int col0 = ct * 16;
-  const double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + col0;
+  const double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + ct0 * 16;
   const uint16_t *__restrict__ Lm = L + loff[m];
   const int rtiles = P2p >> 4;
 
@@ -253,14 +274,13 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
         for (int q = 0; q < 3; ++q)
           pa[par][s][q] = (q < NP) ? a_frag(wave, par, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
   }
-  stage_data<MODE>(bt, ebits, 2 * NSL, [&](int g, int k, int c) {
-    const int par = g / NSL, s = g % NSL, kk = s * 32 + k;
-    const bool v = s < nsl && kk < (par ? ko : ke) && col0 + c < ncol;
-    const double x = sm[(size_t)(v ? 2 * kk + par : 0) * ncol + (v ? c : 0)];
+  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * NG, [&](int g, int k, int c) {
+    const int j = g / NG, par = (g % NG) / NSL, s = g % NSL, kk = s * 32 + k, cc = j * 16 + c;
+    const bool v = s < nsl && kk < (par ? ko : ke) && ct0 * 16 + cc < ncol;
+    const double x = sm[(size_t)(v ? 2 * kk + par : 0) * ncol + (v ? cc : 0)];
     return v ? x : 0.0;
   });
   __syncthreads();
-  const int col = col0 + r16;
   for (int rt = wave; rt < rtiles; rt += 4) {
     if (rt != wave) {
 #pragma unroll
@@ -273,49 +293,56 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     }
     const double ea0 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m]) : 1.0;
     const double ea1 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + 1]) : 1.0;
-    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-    for (int s = 0; s < NSL; ++s) {
-      if (s >= nsl) break;
-      uint4 pb0[3], pb1[3];
-      load_pb<MODE>(bt, s, kq, r16, pb0);
-      load_pb<MODE>(bt, NSL + s, kq, r16, pb1);
-      const double sc0 = Form<MODE>::kScaled ? ea0 * ldexp(1.0, -ebits[s * 16 + r16]) : 1.0;
-      const double sc1 = Form<MODE>::kScaled ? ea1 * ldexp(1.0, -ebits[(NSL + s) * 16 + r16]) : 1.0;
-      slice_mac2<MODE>(pa[0][s], pb0, sc0, acc[0], pa[1][s], pb1, sc1, acc[1]);
-    }
-    if (col < ncol) {
+    for (int j = 0; j < CT; ++j) {
+      if (j >= nc) break;
+      const int g0 = j * NG, col = (ct0 + j) * 16 + r16;
+      d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
 #pragma unroll
-      for (int par = 0; par < 2; ++par)
+      for (int s = 0; s < NSL; ++s) {
+        if (s >= nsl) break;
+        uint4 pb0[3], pb1[3];
+        load_pb<MODE>(bt, g0 + s, kq, r16, pb0);
+        load_pb<MODE>(bt, g0 + NSL + s, kq, r16, pb1);
+        const double sc0 = Form<MODE>::kScaled ? ea0 * ldexp(1.0, -ebits[(g0 + s) * 16 + r16]) : 1.0;
+        const double sc1 =
+            Form<MODE>::kScaled ? ea1 * ldexp(1.0, -ebits[(g0 + NSL + s) * 16 + r16]) : 1.0;
+        slice_mac2<MODE>(pa[0][s], pb0, sc0, acc[0], pa[1][s], pb1, sc1, acc[1]);
+      }
+      if (col < ncol) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int p = rt * 16 + kq * 4 + i;
-          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[par][i];
-        }
+        for (int par = 0; par < 2; ++par)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int p = rt * 16 + kq * 4 + i;
+            if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[par][i];
+          }
+      }
     }
   }
 }
 
-// Forward: four (FP32 S/A halves) -> spec (FP64).  Workgroup = one (m,
-// 16-column tile); the S/A tile (all latitude pairs, both parities) is split
-// once into LDS; wave w takes the (parity, 16-degree row tile) units w, w+4, ...
-template <int MODE, int NSL>
+// Forward: four (FP32 S/A halves) -> spec (FP64).  Workgroup = one (m, CT
+// 16-column tiles); the S/A tiles (all latitude pairs, both parities) are
+// split once into LDS; wave w takes the (parity, 16-degree row tile) units w,
+// w+4, ... and runs each against every column tile with the same table
+// fragments.  NSL = P2p32 / 32 exactly (the dispatch picks it from the plan).
+template <int MODE, int NSL, int CT>
 __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
     const float *__restrict__ four, double *__restrict__ spec, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
     int mlen, int N, int P2, int P2p32, int ncol, int nct, FourGeom geo) {
-  constexpr int NP = Form<MODE>::NP;
-  __shared__ __align__(16) uint16_t bt[2 * NSL * NP * 16 * kRowPitch];
-  __shared__ int ebits[2 * NSL * 16];
-  int m, ct;
-  if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
+  constexpr int NP = Form<MODE>::NP, NG = 2 * NSL;  // LDS groups per column tile
+  __shared__ __align__(16) uint16_t bt[CT * NG * NP * 16 * kRowPitch];
+  __shared__ int ebits[CT * NG * 16];
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
+  const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
   const int M1 = N + 1;
   const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
-  const int nsl = P2p32 >> 5;
-  const int col0 = ct * 16;
-  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + col0;
+  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + ct0 * 16;
   const uint16_t *__restrict__ Lm = L + loff[m];
   const int nunits = 2 * nrt;
 
@@ -329,13 +356,12 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
 #pragma unroll
     for (int s = 0; s < NSL; ++s)
 #pragma unroll
-      for (int q = 0; q < 3; ++q)
-        pa[s][q] = (q < NP) ? a_frag(wave, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
+      for (int q = 0; q < 3; ++q) pa[s][q] = (q < NP) ? a_frag(wave, s, q) : make_uint4(0, 0, 0, 0);
   }
-  stage_data<MODE>(bt, ebits, 2 * nsl, [&](int g, int k, int c) {
-    const int par = g / nsl, s = g % nsl, p = s * 32 + k;
-    const bool v = p < P2 && col0 + c < ncol;
-    const float x = four[geo.at(v ? p : 0, m, par, col0 + (v ? c : 0))];
+  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * NG, [&](int g, int k, int c) {
+    const int j = g / NG, par = (g % NG) / NSL, s = g % NSL, p = s * 32 + k, cc = ct0 * 16 + j * 16 + c;
+    const bool v = p < P2 && cc < ncol;
+    const float x = four[geo.at(v ? p : 0, m, par, v ? cc : ct0 * 16)];
     return v ? (double)x : 0.0;
   });
   __syncthreads();
@@ -346,24 +372,28 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
 #pragma unroll
       for (int s = 0; s < NSL; ++s)
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
-          pa[s][q] = (q < NP) ? a_frag(u, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
+        for (int q = 0; q < 3; ++q) pa[s][q] = (q < NP) ? a_frag(u, s, q) : make_uint4(0, 0, 0, 0);
     }
     const double ea = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + par]) : 1.0;
-    d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int s = 0; s < NSL; ++s) {
-      if (s >= nsl) break;
-      const int g = par * nsl + s;
-      uint4 pb[3];
-      load_pb<MODE>(bt, g, kq, r16, pb);
-      slice_mac<MODE>(pa[s], pb, Form<MODE>::kScaled ? ea * ldexp(1.0, -ebits[g * 16 + r16]) : 1.0, acc);
-    }
-    if (col0 + r16 < ncol) {
+    for (int j = 0; j < CT; ++j) {
+      if (j >= nc) break;
+      d4 acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = rt * 16 + kq * 4 + i;
-        if (k < kc) sm[(size_t)(2 * k + par) * ncol + r16] = acc[i];
+      for (int s = 0; s < NSL; ++s) {
+        const int g = j * NG + par * NSL + s;
+        uint4 pb[3];
+        load_pb<MODE>(bt, g, kq, r16, pb);
+        slice_mac<MODE>(pa[s], pb, Form<MODE>::kScaled ? ea * ldexp(1.0, -ebits[g * 16 + r16]) : 1.0,
+                        acc);
+      }
+      if ((ct0 + j) * 16 + r16 < ncol) {
+        double *__restrict__ sj = sm + j * 16 + r16;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = rt * 16 + kq * 4 + i;
+          if (k < kc) sj[(size_t)(2 * k + par) * ncol] = acc[i];
+        }
       }
     }
   }
@@ -640,10 +670,12 @@ int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, h
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }
-  launch_kernel(leg_inv_mixed_kernel<MODE, NSL>, xcd_m_blocks(p, 0, nct), 256, 0, s,
-      spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
-      p.xcd_mlen[0], p.N, p.P2, p.P2p,
-      ncol, nct, four_geom(p.N, p.im, nf));
+  // KSPLIT: one column tile per workgroup; TILE: two
+  auto kern = shape == SHT_SHAPE_TILE ? leg_inv_mixed_kernel<MODE, NSL, 2> : leg_inv_mixed_kernel<MODE, NSL, 1>;
+  const int ct = shape == SHT_SHAPE_TILE ? 2 : 1;
+  launch_kernel(kern, xcd_m_blocks(p, 0, (nct + ct - 1) / ct), 256, 0, s, spec, four,
+                static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
+                p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -670,10 +702,11 @@ int fwd_nsl(const Plan &p, int shape, const float *four, double *spec, int nf, h
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }
-  launch_kernel(leg_fwd_mixed_kernel<MODE, NSL>, xcd_m_blocks(p, 1, nct), 256, 0, s,
-      four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
-      p.xcd_mlen[1], p.N, p.P2,
-      roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));
+  auto kern = shape == SHT_SHAPE_TILE ? leg_fwd_mixed_kernel<MODE, NSL, 2> : leg_fwd_mixed_kernel<MODE, NSL, 1>;
+  const int ct = shape == SHT_SHAPE_TILE ? 2 : 1;
+  launch_kernel(kern, xcd_m_blocks(p, 1, (nct + ct - 1) / ct), 256, 0, s, four, spec,
+                static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
+                p.xcd_mlen[1], p.N, p.P2, roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }

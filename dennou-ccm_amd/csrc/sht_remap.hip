@@ -7,8 +7,8 @@
 // The triplets are regrouped at creation by receiving point (CSR), keeping
 // the original order within each row, and one thread sums one (row, field)
 // with separately rounded multiply and add in that order: the same operations
-// in the same order as the reference loop, so results are bitwise equal to it
-// (no atomics, no fused multiply-add).
+// in the same order as the reference loop, so results are bitwise equal to an
+// unfused evaluation of it (no atomics, no fused multiply-add).
 #include <algorithm>
 #include <new>
 #include <numeric>
@@ -31,12 +31,19 @@ __global__ __launch_bounds__(256) void remap_kernel(const int *__restrict__ row,
                                                     const double *__restrict__ send,
                                                     double *__restrict__ recv, int nsend, int nrecv,
                                                     int ndata) {
+  // separately rounded multiply and add: hipcc otherwise contracts the pair
+  // into one fused multiply-add (the operations must be written here, in the
+  // pragma's scope: __dmul_rn/__dadd_rn are contracted at their definition)
+#pragma clang fp contract(off)
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nrecv * ndata) return;
   const int r = (int)(t % nrecv), d = (int)(t / nrecv);
   const double *__restrict__ s = send + (size_t)d * nsend;
   double acc = 0.0;
-  for (int i = row[r]; i < row[r + 1]; ++i) acc = __dadd_rn(acc, __dmul_rn(s[col[i]], val[i]));
+  for (int i = row[r]; i < row[r + 1]; ++i) {
+    const double prod = s[col[i]] * val[i];
+    acc = acc + prod;
+  }
   recv[(size_t)d * nrecv + r] = acc;
 }
 

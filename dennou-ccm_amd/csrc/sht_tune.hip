@@ -17,8 +17,9 @@ namespace sht {
 namespace {
 
 // FP64 plans: every shape; mixed-precision plans: KSPLIT (shared LDS data
-// tile) and WAVE (data tile per wave in registers).
-int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_NSHAPE : 2; }
+// tile of one column tile), WAVE (data tile per wave in registers) and TILE
+// (shared LDS data tiles of two column tiles).
+int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_NSHAPE : 3; }
 
 bool shape_ok(const Plan &p, int s) { return s == SHT_SHAPE_AUTO || (s >= 0 && s < nshapes(p)); }
 

@@ -267,8 +267,12 @@ enum {
   SHT_NSHAPE = 4
 };
 /* Fixes the shapes for calls with nf fields (nf = 0: the plan default for every
- * count without an entry).  FP64 plans take every shape; mixed-precision plans
- * take SHT_SHAPE_AUTO and SHT_SHAPE_KSPLIT. */
+ * count without an entry).  FP64 plans take every shape.  Mixed-precision
+ * plans take SHT_SHAPE_AUTO, SHT_SHAPE_KSPLIT (workgroup per (m, 16 columns),
+ * data tile shared in LDS), SHT_SHAPE_WAVE (wavefront per 16 columns, data
+ * tile in registers) and SHT_SHAPE_TILE (workgroup per (m, 32 columns), each
+ * wave's table fragments used for both column tiles); their results are
+ * bitwise equal. */
 int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
 /* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
 int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);

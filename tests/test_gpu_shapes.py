@@ -178,36 +178,40 @@ def test_shape_argument_checks():
     p.close()
     m = shtamd.Plan(21, 64, 32, 8, precision=shtamd.PREC_SPLIT_BF16X6)
     with pytest.raises(shtamd.ShtError):
-        m.set_shapes(0, shtamd.SHAPE_TILE, shtamd.SHAPE_TILE)
+        m.set_shapes(0, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_TILE_SPLIT)
     m.set_shapes(0, shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     assert m.get_shapes(5) == (shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     (inv, fwd), _ = m.autotune(8, reps=2)
-    assert inv in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE) and fwd in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE)
+    assert inv in MIXED_SHAPES and fwd in MIXED_SHAPES
     m.close()
 
 
+MIXED_SHAPES = (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE)
 MIXED_TOL = {shtamd.PREC_SPLIT_BF16: 5e-5, shtamd.PREC_SCALED_FP16: 1e-6, shtamd.PREC_SPLIT_BF16X6: 1e-6}
 
 
 @pytest.mark.parametrize("prec", sorted(MIXED_TOL))
 @pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (21, 64, 32, 9), (170, 512, 256, 7), (31, 64, 48, 17)])
 def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
-    """The two mixed-precision shapes split the same operands into the same
-    pieces and sum the same products in the same order: bitwise equal results,
-    within the form's bound of the oracle."""
+    """The mixed-precision shapes (one or two column tiles per workgroup, or a
+    wave per tile) split the same operands into the same pieces and sum the
+    same products in the same order: bitwise equal results, within the form's
+    bound of the oracle.  The field counts give odd and even numbers of
+    16-column tiles, so the two-tile shape also runs a lone last tile."""
     say("mixed shapes", prec, N, nf)
     spec = synth.spectra(N, nf, cfg=14)
     ref = oracle.Plan(N, im, jm)
     g_ref = ref.spec_to_grid(spec)
     p = shtamd.Plan(N, im, jm, nf, precision=prec)
     out = []
-    for sh in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE):
+    for sh in MIXED_SHAPES:
         p.set_shapes(0, sh, sh)
         g = run_inverse(p, spec, nf)[0]
         s = run_forward(p, g_ref, nf)
         assert rel(g, g_ref) < MIXED_TOL[prec]
         assert rel(s, spec) < 2 * MIXED_TOL[prec]
         out.append((g, s))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    for g, s in out[1:]:
+        assert np.array_equal(out[0][0], g)
+        assert np.array_equal(out[0][1], s)
     p.close()
