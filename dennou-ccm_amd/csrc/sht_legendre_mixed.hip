@@ -170,9 +170,10 @@ constexpr int kRowPitch = 40;
 // lane writes each of its pieces as one 16-byte LDS store -- already the
 // matrix-core B layout.  A wave issues the loads of NIT groups before it
 // converts any, so one memory latency covers them (groups past ngroups load
-// group 0 again and are not stored).
-template <int MODE, int NIT, class Load>
-__device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Load load) {
+// group 0 again and are not stored).  Group g lands in LDS slot slot(g), so a
+// caller can stage only the groups that hold data.
+template <int MODE, int NIT, class Slot, class Load>
+__device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Slot slot, Load load) {
   constexpr int NP = Form<MODE>::NP;
   const int lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
   for (int g0 = wave_id(); g0 < ngroups; g0 += 4 * NIT) {
@@ -185,8 +186,8 @@ __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      const int g = g0 + 4 * it;
-      if (g >= ngroups) break;
+      if (g0 + 4 * it >= ngroups) break;
+      const int g = slot(g0 + 4 * it);
       int e = 0;
       if constexpr (Form<MODE>::kScaled) {
         double mx = 0.0;
@@ -230,6 +231,10 @@ __device__ __forceinline__ void load_pb(const uint16_t *bt, int g, int kq, int r
                      : make_uint4(0, 0, 0, 0);
 }
 
+// Row pitch (floats) of the inverse's output transpose: 32 values of a pair
+// row plus 4, so the 4 lane groups of one ds_write hit different banks.
+constexpr int kOutPitch = 36;
+
 // NSL (template) = the kernel's K slices of 32 at m = 0: KP32/32 <= 4 for the
 // inverse (N <= 255), P2p32/32 <= 8 for the forward (jm <= 512).  Registers and
 // LDS are sized by it, so small truncations keep a high occupancy.
@@ -240,7 +245,9 @@ __device__ __forceinline__ void load_pb(const uint16_t *bt, int g, int kq, int r
 // spectral tiles into LDS; wave w then takes row tiles w, w+4, ... and runs
 // each against every column tile with the same table fragments.  CT = 2
 // halves the workgroups, and with them the waves that wait on memory at
-// once, and the table fragments fetched per column.
+// once, and the table fragments fetched per column.  Results leave through a
+// per-wave LDS transpose as 16-byte stores of 4 consecutive columns (the
+// Fourier layout keeps a field group's 2G columns contiguous).
 template <int MODE, int NSL, int CT>
 __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
@@ -249,6 +256,7 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
   constexpr int NP = Form<MODE>::NP, NG = 2 * NSL;  // LDS groups per column tile
   __shared__ __align__(16) uint16_t bt[CT * NG * NP * 16 * kRowPitch];
   __shared__ int ebits[CT * NG * 16];
+  __shared__ __align__(16) float obuf[4 * 16 * kOutPitch];  // per-wave output transpose
   int m, t;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
   const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
@@ -264,33 +272,35 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     return *reinterpret_cast<const uint4 *>(
         Lm + ((size_t)(par * NP + q) * P2p + rt * 16 + r16) * KP32 + s * 32 + kq * 8);
   };
+  // table fragments of the nsl slices in use (the others stay zero, unused)
   uint4 pa[2][NSL][3];
-  if (wave < rtiles) {
+  auto load_a = [&](int rt) {
 #pragma unroll
     for (int par = 0; par < 2; ++par)
 #pragma unroll
       for (int s = 0; s < NSL; ++s)
 #pragma unroll
         for (int q = 0; q < 3; ++q)
-          pa[par][s][q] = (q < NP) ? a_frag(wave, par, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
-  }
-  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * NG, [&](int g, int k, int c) {
-    const int j = g / NG, par = (g % NG) / NSL, s = g % NSL, kk = s * 32 + k, cc = j * 16 + c;
-    const bool v = s < nsl && kk < (par ? ko : ke) && ct0 * 16 + cc < ncol;
+          pa[par][s][q] = (q < NP && s < nsl) ? a_frag(rt, par, s, q) : make_uint4(0, 0, 0, 0);
+  };
+  if (wave < rtiles) load_a(wave);
+  // only the nsl slices that hold degrees: group g = (j, parity, s) of
+  // column tile j goes to slot j NG + parity NSL + s
+  const int ng = 2 * nsl;
+  auto slot = [&](int g) {
+    const int j = g / ng, r = g - j * ng, par = r >= nsl;
+    return j * NG + par * NSL + r - par * nsl;
+  };
+  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * ng, slot, [&](int g, int k, int c) {
+    const int j = g / ng, r = g - j * ng, par = r >= nsl, s = r - par * nsl;
+    const int kk = s * 32 + k, cc = j * 16 + c;
+    const bool v = kk < (par ? ko : ke) && ct0 * 16 + cc < ncol;
     const double x = sm[(size_t)(v ? 2 * kk + par : 0) * ncol + (v ? cc : 0)];
     return v ? x : 0.0;
   });
   __syncthreads();
   for (int rt = wave; rt < rtiles; rt += 4) {
-    if (rt != wave) {
-#pragma unroll
-      for (int par = 0; par < 2; ++par)
-#pragma unroll
-        for (int s = 0; s < NSL; ++s)
-#pragma unroll
-          for (int q = 0; q < 3; ++q)
-            pa[par][s][q] = (q < NP) ? a_frag(rt, par, s < nsl ? s : 0, q) : make_uint4(0, 0, 0, 0);
-    }
+    if (rt != wave) load_a(rt);
     const double ea0 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m]) : 1.0;
     const double ea1 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + 1]) : 1.0;
 #pragma unroll
@@ -309,7 +319,24 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
             Form<MODE>::kScaled ? ea1 * ldexp(1.0, -ebits[(g0 + NSL + s) * 16 + r16]) : 1.0;
         slice_mac2<MODE>(pa[0][s], pb0, sc0, acc[0], pa[1][s], pb1, sc1, acc[1]);
       }
-      if (col < ncol) {
+      if (geo.lg >= 2) {
+        // 4 consecutive columns per lane: through this wave's LDS patch
+        // [16 pairs][2 parities][16 columns] (row pitch kOutPitch); the
+        // padding columns of the last field group are written too (zeros)
+        float *ob = obuf + wave * (16 * kOutPitch);
+#pragma unroll
+        for (int par = 0; par < 2; ++par)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) ob[(kq * 4 + i) * kOutPitch + par * 16 + r16] = (float)acc[par][i];
+        const int cend = geo.nfg << geo.lg;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int q = lane + 64 * h, pl = q >> 3, par = (q >> 2) & 1, c4 = (q & 3) * 4;
+          const int p = rt * 16 + pl, cc = (ct0 + j) * 16 + c4;
+          const float4 v = *reinterpret_cast<const float4 *>(ob + pl * kOutPitch + par * 16 + c4);
+          if (p < P2 && cc < cend) *reinterpret_cast<float4 *>(four + geo.at(p, m, par, cc)) = v;
+        }
+      } else if (col < ncol) {  // field groups of fewer than 4 columns (im = 2048)
 #pragma unroll
         for (int par = 0; par < 2; ++par)
 #pragma unroll
@@ -358,7 +385,7 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
 #pragma unroll
       for (int q = 0; q < 3; ++q) pa[s][q] = (q < NP) ? a_frag(wave, s, q) : make_uint4(0, 0, 0, 0);
   }
-  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * NG, [&](int g, int k, int c) {
+  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * NG, [](int g) { return g; }, [&](int g, int k, int c) {
     const int j = g / NG, par = (g % NG) / NSL, s = g % NSL, p = s * 32 + k, cc = ct0 * 16 + j * 16 + c;
     const bool v = p < P2 && cc < ncol;
     const float x = four[geo.at(v ? p : 0, m, par, v ? cc : ct0 * 16)];
