@@ -213,6 +213,7 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
     return fail(SHT_ERR_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
   }
   rc = build_tables(p);
+  if (!rc) p.fft_fwd_resident = fft_fwd_capacity(p);
   if (!rc && p.prec != SHT_PREC_FP64) rc = build_mixed_tables(p);
   if (!rc) {
     // [P2][nfg][N+1][2][2G]: field groups of G = 2048/im are allocated whole

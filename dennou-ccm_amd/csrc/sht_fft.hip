@@ -16,8 +16,13 @@
 // (m,lat) transpose: global traffic is row-contiguous on the grid side and
 // 2G-double runs on the Fourier side; the reordering happens in LDS.
 // The complex FFT is a Stockham autosort with radix 8/16 butterflies held in
-// registers (two or three LDS passes per transform) in a single 32 KB LDS
-// buffer holding G*L = 2048 complex doubles.
+// registers (two or three passes per transform) over a single 32 KB LDS
+// buffer holding G*L = 2048 complex doubles.  LDS bandwidth bounds the
+// passes, so the ends skip what they can: the inverse places only the
+// coefficients m < M1 and L - m, reads the band between them as zeros, and
+// stores its last pass straight to the grid; the forward writes only those
+// wavenumbers in its last pass and, when the launch fits the device at once,
+// feeds its first pass straight from the grid.
 #include "sht_internal.h"
 
 namespace sht {
@@ -92,13 +97,15 @@ __device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
   for (int i = 0; i < R; ++i) v[i] = t[i];
 }
 
-// One Stockham pass of radix R over G transforms of length L in buf, Ns the
-// product of the radices already applied.  Butterfly j of a transform reads
-// x[j + r L/R], applies w^(r k L/(Ns R)) (k = j mod Ns; the powers are built by
-// repeated multiplication from one LDS twiddle), runs a register DFT and
-// writes y[(j/Ns) Ns R + k + r Ns].  Reads finish (barrier) before writes.
-template <int L, int R, int Ns, int SIGN>
-__device__ __forceinline__ void stockham_pass(double2 *buf, const double2 *tw) {
+// One Stockham pass of radix R over G transforms of length L, Ns the product
+// of the radices already applied.  Butterfly j of transform g reads
+// x[j + r L/R] (in(g, i) returns element i of transform g), applies
+// w^(r k L/(Ns R)) (k = j mod Ns; the powers are built by repeated
+// multiplication from one LDS twiddle), runs a register DFT and writes
+// y[(j/Ns) Ns R + k + r Ns] through out(g, i, v).  A barrier separates the
+// reads from the writes (the passes run in place in buf) and follows them.
+template <int L, int R, int Ns, int SIGN, class In, class Out>
+__device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out) {
   constexpr int G = kLdsComplex / L;
   constexpr int NB = G * (L / R);                       // butterflies in the block
   constexpr int QB = (NB + kThreads - 1) / kThreads;    // per thread
@@ -110,7 +117,7 @@ __device__ __forceinline__ void stockham_pass(double2 *buf, const double2 *tw) {
     if (NB % kThreads != 0 && idx >= NB) continue;
     const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[q][r] = buf[px<L>(g * L + j + r * (L / R))];
+    for (int r = 0; r < R; ++r) v[q][r] = in(g, j + r * (L / R));
     if constexpr (Ns > 1) {
       double2 w1 = tw[k * (L / (Ns * R))];
       w1.y *= SIGN;
@@ -129,47 +136,52 @@ __device__ __forceinline__ void stockham_pass(double2 *buf, const double2 *tw) {
     const int idx = tid + q * kThreads;
     if (NB % kThreads != 0 && idx >= NB) continue;
     const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
-    const int d0 = g * L + (j / Ns) * Ns * R + k;
+    const int d0 = (j / Ns) * Ns * R + k;
 #pragma unroll
-    for (int r = 0; r < R; ++r) buf[px<L>(d0 + r * Ns)] = v[q][r];
+    for (int r = 0; r < R; ++r) out(g, d0 + r * Ns, v[q][r]);
   }
   __syncthreads();
 }
 
-// In-place complex FFT of G = kLdsComplex / L transforms stored back to back in
-// buf; exponent sign SIGN (-1 forward, +1 inverse), unnormalised.  tw (in LDS)
-// holds e^{+2 pi i t / L}.  Radix plan: 16 = 16, 32 = 8*4, 64 = 8*8,
-// 128 = 16*8, 256 = 16*16, 512 = 8*8*8, 1024 = 16*16*4, 2048 = 16*16*8.
-// Ends with a barrier.
-template <int L, int SIGN>
-__device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw) {
+// Complex FFT of G = kLdsComplex / L transforms, exponent sign SIGN (-1
+// forward, +1 inverse), unnormalised; tw (in LDS) holds e^{+2 pi i t / L}.
+// The first pass reads through first(g, i) and the last writes through
+// last(g, i, v), so a caller can feed the first butterflies straight from
+// global memory, skip known zeros, or store the results straight out, and
+// save the LDS round trips; the passes between run in place in buf.
+// Radix plan: 16 = 16, 32 = 8*4, 64 = 8*8, 128 = 16*8, 256 = 16*16,
+// 512 = 8*8*8, 1024 = 16*16*4, 2048 = 16*16*8.  Ends with a barrier.
+template <int L, int SIGN, class First, class Last>
+__device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw, First first, Last last) {
+  auto rd = [&](int g, int i) { return buf[px<L>(g * L + i)]; };
+  auto wr = [&](int g, int i, double2 v) { buf[px<L>(g * L + i)] = v; };
   if constexpr (L == 16) {
-    stockham_pass<L, 16, 1, SIGN>(buf, tw);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, last);
   } else if constexpr (L == 32) {
-    stockham_pass<L, 8, 1, SIGN>(buf, tw);
-    stockham_pass<L, 4, 8, SIGN>(buf, tw);
+    stockham_pass<L, 8, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 4, 8, SIGN>(tw, rd, last);
   } else if constexpr (L == 64) {
-    stockham_pass<L, 8, 1, SIGN>(buf, tw);
-    stockham_pass<L, 8, 8, SIGN>(buf, tw);
+    stockham_pass<L, 8, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 8, 8, SIGN>(tw, rd, last);
   } else if constexpr (L == 128) {
-    stockham_pass<L, 16, 1, SIGN>(buf, tw);
-    stockham_pass<L, 8, 16, SIGN>(buf, tw);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 8, 16, SIGN>(tw, rd, last);
   } else if constexpr (L == 256) {
-    stockham_pass<L, 16, 1, SIGN>(buf, tw);
-    stockham_pass<L, 16, 16, SIGN>(buf, tw);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 16, 16, SIGN>(tw, rd, last);
   } else if constexpr (L == 512) {
-    stockham_pass<L, 8, 1, SIGN>(buf, tw);
-    stockham_pass<L, 8, 8, SIGN>(buf, tw);
-    stockham_pass<L, 8, 64, SIGN>(buf, tw);
+    stockham_pass<L, 8, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 8, 8, SIGN>(tw, rd, wr);
+    stockham_pass<L, 8, 64, SIGN>(tw, rd, last);
   } else if constexpr (L == 1024) {
-    stockham_pass<L, 16, 1, SIGN>(buf, tw);
-    stockham_pass<L, 16, 16, SIGN>(buf, tw);
-    stockham_pass<L, 4, 256, SIGN>(buf, tw);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 16, 16, SIGN>(tw, rd, wr);
+    stockham_pass<L, 4, 256, SIGN>(tw, rd, last);
   } else {
     static_assert(L == 2048, "unsupported ring length");
-    stockham_pass<L, 16, 1, SIGN>(buf, tw);
-    stockham_pass<L, 16, 16, SIGN>(buf, tw);
-    stockham_pass<L, 8, 256, SIGN>(buf, tw);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr);
+    stockham_pass<L, 16, 16, SIGN>(tw, rd, wr);
+    stockham_pass<L, 8, 256, SIGN>(tw, rd, last);
   }
 }
 
@@ -181,9 +193,13 @@ struct FwdIO {
 };
 
 // grid -> four (S, A halves).  Blocks: P2 * ceil(nf / G) x inputs.
-// All global loads of a thread (its 2 * G*L/256 ring samples and its twiddles)
-// are issued before the first LDS store, so a block pays one HBM latency.
-template <int L, typename TF>
+// DIRECT: the first FFT pass reads its butterflies' ring samples straight
+// from global memory (runs of L/R samples), which saves an LDS round trip and
+// wins while every block of the launch is resident at once; otherwise all
+// threads first load whole rows into LDS (runs of 64 samples per wave), which
+// streams better when blocks queue.  The last pass writes only the
+// wavenumbers m < M1 and L - m the Legendre step reads.
+template <int L, typename TF, bool DIRECT>
 __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
                                                            const double2 *__restrict__ tw,
                                                            const double *__restrict__ coslat,
@@ -192,7 +208,6 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
   TF *__restrict__ four = io.dst[blockIdx.y];
   const int mode = io.mode[blockIdx.y];
   constexpr int G = kLdsComplex / L;
-  constexpr int NQ = kLdsComplex / kThreads; // samples per thread
   __shared__ double2 buf[kLdsPadded];
   __shared__ double2 twl[L];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -200,22 +215,35 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
   const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const int tid = threadIdx.x;
-  double2 z[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int idx = tid + q * kThreads;
-    const int g = idx / L, i = idx % L, f = f0 + g;
-    z[q] = make_double2(0.0, 0.0);
-    if (f < nf) {
-      z[q].x = __builtin_nontemporal_load(grid + ((size_t)f * jm + jn) * L + i);
-      z[q].y = __builtin_nontemporal_load(grid + ((size_t)f * jm + js) * L + i);
-    }
-  }
   for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
+  // z = f_N + i f_S; transforms past nf are zero
+  auto ring = [&](int g, int i) {
+    const int f = f0 + g;
+    double2 z = make_double2(0.0, 0.0);
+    if (f < nf) {
+      z.x = __builtin_nontemporal_load(grid + ((size_t)f * jm + jn) * L + i);
+      z.y = __builtin_nontemporal_load(grid + ((size_t)f * jm + js) * L + i);
+    }
+    return z;
+  };
+  auto keep = [&](int g, int i, double2 v) {
+    if (i < M1 || i > L - M1) buf[px<L>(g * L + i)] = v;
+  };
+  if constexpr (DIRECT) {
+    fft_lds<L, -1>(buf, twl, ring, keep);
+  } else {
+    constexpr int NQ = kLdsComplex / kThreads;  // samples per thread
+    double2 z[NQ];
 #pragma unroll
-  for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = z[q];
-  __syncthreads();
-  fft_lds<L, -1>(buf, twl);
+    for (int q = 0; q < NQ; ++q) {
+      const int idx = tid + q * kThreads;
+      z[q] = ring(idx / L, idx % L);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = z[q];
+    __syncthreads();
+    fft_lds<L, -1>(buf, twl, [&](int g, int i) { return buf[px<L>(g * L + i)]; }, keep);
+  }
   const double sc = 0.5 / L * (mode == 0 ? 1.0 : 1.0 / coslat[jn]);
   for (int idx = tid; idx < M1 * G; idx += kThreads) {
     const int m = idx / G, g = idx % G, f = f0 + g;
@@ -247,14 +275,16 @@ struct InvOutputs {
 
 // four (even/odd Legendre sums) -> grid.  Blocks: P2 * ceil(nf / G) x noutputs.
 // A thread's Fourier loads (at most 4 (m, field) pairs: M1 * G <= L/2 * G =
-// 1024) and its twiddles are issued before any LDS traffic.
+// 1024) and its twiddles are issued before any LDS traffic.  Only the
+// coefficients m < M1 and L - m are placed in LDS: the first FFT pass takes
+// the band between them as zeros without reading it, and the last pass
+// stores the rings straight to the grid (non-temporal).
 template <int L, typename TF>
 __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
                                                            const double2 *__restrict__ tw,
                                                            const double *__restrict__ coslat,
                                                            int P2, int M1, int nf, int jm) {
   constexpr int G = kLdsComplex / L;
-  constexpr int NQ = kLdsComplex / kThreads;
   constexpr int NM = (kLdsComplex / 2) / kThreads; // max (m, field) pairs per thread
   __shared__ double2 buf[kLdsPadded];
   __shared__ double2 twl[L];
@@ -280,10 +310,7 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
       O[q] = make_double2((double)e[2 * G], (double)e[2 * G + 1]);
     }
   }
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = make_double2(0.0, 0.0);
   for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
-  __syncthreads();
 #pragma unroll
   for (int q = 0; q < NM; ++q) {
     const int idx = tid + q * kThreads;
@@ -306,26 +333,39 @@ __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
     }
   }
   __syncthreads();
-  fft_lds<L, +1>(buf, twl);
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) {
-    const int idx = tid + q * kThreads;
-    const int g = idx / L, i = idx % L, f = f0 + g;
-    if (f >= nf) continue;
-    const double2 z = buf[px<L>(idx)];
-    __builtin_nontemporal_store(z.x, dst + ((size_t)f * jm + jn) * L + i);
-    __builtin_nontemporal_store(z.y, dst + ((size_t)f * jm + js) * L + i);
-  }
+  // transforms past nf hold stale LDS values: their rings are not stored
+  auto spectrum = [&](int g, int i) {
+    return (i < M1 || i > L - M1) ? buf[px<L>(g * L + i)] : make_double2(0.0, 0.0);
+  };
+  auto rings = [&](int g, int i, double2 z) {
+    const int f = f0 + g;
+    if (f < nf) {
+      __builtin_nontemporal_store(z.x, dst + ((size_t)f * jm + jn) * L + i);
+      __builtin_nontemporal_store(z.y, dst + ((size_t)f * jm + js) * L + i);
+    }
+  };
+  fft_lds<L, +1>(buf, twl, spectrum, rings);
 }
 
 template <int L, typename TF>
 int fwd_dispatch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  launch_kernel(fft_fwd_kernel<L, TF>, dim3(nblk, nin), kThreads, 0, s,
+  auto kern = nblk * nin <= p.fft_fwd_resident ? fft_fwd_kernel<L, TF, true> : fft_fwd_kernel<L, TF, false>;
+  launch_kernel(kern, dim3(nblk, nin), kThreads, 0, s,
       io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
+}
+
+template <int L>
+int fwd_resident(int device) {
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_fwd_kernel<L, double, true>, kThreads, 0) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    return 0;
+  return per_cu * ncu;
 }
 
 template <int L, typename TF>
@@ -377,6 +417,20 @@ int inv_typed(const Plan &p, const TF *four, const TF *four2, double *grid, doub
 }
 
 }  // namespace
+
+int fft_fwd_capacity(const Plan &p) {
+  switch (p.im) {
+    case 16: return fwd_resident<16>(p.device);
+    case 32: return fwd_resident<32>(p.device);
+    case 64: return fwd_resident<64>(p.device);
+    case 128: return fwd_resident<128>(p.device);
+    case 256: return fwd_resident<256>(p.device);
+    case 512: return fwd_resident<512>(p.device);
+    case 1024: return fwd_resident<1024>(p.device);
+    case 2048: return fwd_resident<2048>(p.device);
+  }
+  return 0;
+}
 
 int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s) {
   if (f32) {

@@ -55,6 +55,7 @@ int fail(int code, const std::string &msg);
 struct Plan {
   int N = 0, im = 0, jm = 0, P2 = 0, P2p = 0, K = 0, kmax = 0, prec = 0, flags = 0;
   int device = 0;
+  int fft_fwd_resident = 0;  // forward FFT blocks the device holds at once (0: unknown)
   int maxke = 0;            // ceil((N+1)/2): most rows of one parity
   // device tables
   double *mu = nullptr, *w = nullptr, *coslat = nullptr; // jm each, S->N
@@ -150,6 +151,8 @@ int build_tables(Plan &p);
 
 // FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos).
 // f32 = the Fourier intermediate is FP32 (mixed-precision plans); the FFT itself is FP64.
+// Forward FFT blocks one device holds at once (occupancy x CUs; 0 if unknown).
+int fft_fwd_capacity(const Plan &p);
 int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s);
 // Divergence analysis FFT: a -> fa with x i m / cos, b -> fb with x 1 / cos (FP64;
 // either pair may be NULL), one launch.
