@@ -84,6 +84,27 @@ __device__ __forceinline__ void split(double x, uint16_t (&pc)[3]) {
   }
 }
 
+// The same pieces for an FP32 value (the Fourier data of the forward
+// transform), in FP32 arithmetic: x has 24 significant bits, so every
+// residual is exact in FP32 -- the pieces equal those the FP64 split above
+// gives for (double)x -- and hi + mid + lo = x exactly for BF16X6.
+__device__ __forceinline__ uint16_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+__device__ __forceinline__ float bf16_valf(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+template <int MODE>
+__device__ __forceinline__ void split(float x, uint16_t (&pc)[3]) {
+  if constexpr (MODE == SHT_PREC_SCALED_FP16) {
+    const _Float16 h = (_Float16)x;
+    pc[0] = __builtin_bit_cast(uint16_t, h);
+    pc[1] = __builtin_bit_cast(uint16_t, (_Float16)((x - (float)h) * 2048.0f));
+    pc[2] = 0;
+  } else {
+    pc[0] = bf16_bits(x);
+    const float r = x - bf16_valf(pc[0]);
+    pc[1] = bf16_bits(r);
+    pc[2] = bf16_bits(r - bf16_valf(pc[1]));
+  }
+}
+
 template <int MODE>
 __device__ __forceinline__ f4 mfma16(const uint4 &a, const uint4 &b, f4 c) {
   if constexpr (MODE == SHT_PREC_SCALED_FP16)
@@ -176,8 +197,9 @@ template <int MODE, int NIT, class Slot, class Load>
 __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Slot slot, Load load) {
   constexpr int NP = Form<MODE>::NP;
   const int lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
+  using T = decltype(load(0, 0, 0));  // double (spectra) or float (Fourier data)
   for (int g0 = wave_id(); g0 < ngroups; g0 += 4 * NIT) {
-    double v[NIT][8];
+    T v[NIT][8];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
       const int g = g0 + 4 * it < ngroups ? g0 + 4 * it : 0;
@@ -190,12 +212,12 @@ __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups
       const int g = slot(g0 + 4 * it);
       int e = 0;
       if constexpr (Form<MODE>::kScaled) {
-        double mx = 0.0;
+        T mx = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) mx = fmax(mx, fabs(v[it][i]));
         mx = fmax(mx, __shfl_xor(mx, 16));
         mx = fmax(mx, __shfl_xor(mx, 32));
-        e = mx > 0.0 ? 13 - ilogb(mx) : 0;
+        e = mx > 0 ? 13 - ilogb(mx) : 0;
         if (k8 == 0) ebits[g * 16 + c] = e;
       }
       uint32_t w[3][4];
@@ -389,7 +411,7 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
     const int j = g / NG, par = (g % NG) / NSL, s = g % NSL, p = s * 32 + k, cc = ct0 * 16 + j * 16 + c;
     const bool v = p < P2 && cc < ncol;
     const float x = four[geo.at(v ? p : 0, m, par, v ? cc : ct0 * 16)];
-    return v ? (double)x : 0.0;
+    return v ? x : 0.0f;
   });
   __syncthreads();
   for (int u = wave; u < nunits; u += 4) {
@@ -433,8 +455,8 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
 // there is no LDS staging and no barrier.  The wave then runs every row tile
 // (inverse: 16 latitude pairs; forward: parity x 16 degrees), loading the
 // table fragments, which the four waves of a workgroup share through L1.
-template <int MODE>
-__device__ __forceinline__ void split8(const double (&v)[8], int e, uint4 (&pb)[3]) {
+template <int MODE, typename T>
+__device__ __forceinline__ void split8(const T (&v)[8], int e, uint4 (&pb)[3]) {
   uint32_t w[3][4];
 #pragma unroll
   for (int i = 0; i < 8; i += 2) {
@@ -448,15 +470,15 @@ __device__ __forceinline__ void split8(const double (&v)[8], int e, uint4 (&pb)[
   for (int q = 0; q < 3; ++q) pb[q] = make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
 }
 
-template <int MODE>
-__device__ __forceinline__ int col_exponent(const double (&v)[8]) {
+template <int MODE, typename T>
+__device__ __forceinline__ int col_exponent(const T (&v)[8]) {
   if constexpr (!Form<MODE>::kScaled) return 0;
-  double mx = 0.0;
+  T mx = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) mx = fmax(mx, fabs(v[i]));
   mx = fmax(mx, __shfl_xor(mx, 16));
   mx = fmax(mx, __shfl_xor(mx, 32));
-  return mx > 0.0 ? 13 - ilogb(mx) : 0;
+  return mx > 0 ? 13 - ilogb(mx) : 0;
 }
 
 template <int MODE, int NSL>
@@ -556,12 +578,12 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_wave_kernel(
   for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int sl = 0; sl < NSL; ++sl) {
-      double v[8];
+      float v[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int p = sl * 32 + kq * 8 + i;
         const bool ok = sl < nsl && cok && p < P2;
-        v[i] = (double)*(ok ? four + geo.at(p, m, q, csafe) : fz);
+        v[i] = *(ok ? four + geo.at(p, m, q, csafe) : fz);
       }
       const int e = col_exponent<MODE>(v);
       bsc[q][sl] = ldexp(1.0, -e);
