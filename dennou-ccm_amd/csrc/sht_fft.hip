@@ -40,6 +40,8 @@ constexpr int kLdsPadded = kLdsComplex + kLdsComplex / 16 + kLdsComplex / 16;
 template <int L>
 __device__ __forceinline__ int px(int i) { return i + (i >> 4) + i / L; }
 
+// Regular stores: the Legendre step that reads the Fourier data next finds
+// them in the Infinity Cache (streaming them made it slower at every size).
 __device__ __forceinline__ void store_pair(double *o, double2 v) { *reinterpret_cast<double2 *>(o) = v; }
 __device__ __forceinline__ void store_pair(float *o, double2 v) {
   *reinterpret_cast<float2 *>(o) = make_float2((float)v.x, (float)v.y);

@@ -41,7 +41,7 @@ constexpr int kSteps = 4;   // K-steps (of 4) per wave whose fragments are loade
 // issues all their table and spectral fragments at once, runs its MFMAs over
 // RT row tiles (16 latitude pairs each), and the four partial sums are added
 // in FP64 through LDS.
-template <int RT>
+template <int RT, bool NT>
 __global__ __launch_bounds__(256) void leg_inv_f64_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
     const double *__restrict__ Pinv, const double *__restrict__ Hinv,
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void leg_inv_f64_kernel(
         const double v = red[e] + red[e + RT * 256] + red[e + 2 * RT * 256] + red[e + 3 * RT * 256];
         const int ln = e & 63, ri = e >> 6, i = ri & 3, r = ri >> 2;
         const int p = (rt0 + r) * 16 + i * 4 + (ln >> 4), c = ct * 16 + (ln & 15);
-        if (p < P2 && c < ncol) out[geo.at(p, m, slot, c)] = v;
+        if (p < P2 && c < ncol) put_four<NT>(out + geo.at(p, m, slot, c), v);
       }
       __syncthreads();
     }
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
       const int k = r * 16 + i * 4 + (ln >> 4), c = ct * 16 + (ln & 15);
       if (k < kcnt[q] && c < ncol) {
         double *o = sm + (size_t)(2 * k + q) * ncol + c;
-        *o = accum ? *o + v : v;
+        __builtin_nontemporal_store(accum ? *o + v : v, o);
       }
     }
     __syncthreads();
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
 // workgroup share table fragments (same m and row tile) in L1.
 
 // Inverse, unit = (m, table, row tile of 16 pairs, column tile); heaviest m first.
-template <int D>
+template <int D, bool NT>
 __global__ __launch_bounds__(256) void leg_inv_f64_wave_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
     const double *__restrict__ Pinv, const double *__restrict__ Hinv,
@@ -270,8 +270,8 @@ __global__ __launch_bounds__(256) void leg_inv_f64_wave_kernel(
     for (int i = 0; i < 4; ++i) {
       const int p = rt * 16 + i * 4 + kq;
       if (p < P2) {
-        out[geo.at(p, m, se, col)] = acc0[i];
-        out[geo.at(p, m, so, col)] = acc1[i];
+        put_four<NT>(out + geo.at(p, m, se, col), acc0[i]);
+        put_four<NT>(out + geo.at(p, m, so, col), acc1[i]);
       }
     }
   }
@@ -337,8 +337,8 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_wave_kernel(
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int k = kt * 16 + i * 4 + kq;
-      if (k < ke) sm[(size_t)(2 * k) * ncol] = acc0[i];
-      if (k < ko) sm[(size_t)(2 * k + 1) * ncol] = acc1[i];
+      if (k < ke) __builtin_nontemporal_store(acc0[i], sm + (size_t)(2 * k) * ncol);
+      if (k < ko) __builtin_nontemporal_store(acc1[i], sm + (size_t)(2 * k + 1) * ncol);
     }
   }
 }
@@ -357,7 +357,7 @@ struct KSplit {
   int m4, m2;
 };
 
-template <int RT, int D>
+template <int RT, int D, bool NT>
 __global__ __launch_bounds__(256) void leg_inv_f64_tile_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
     const double *__restrict__ Pinv, const double *__restrict__ Hinv,
@@ -442,8 +442,8 @@ __global__ __launch_bounds__(256) void leg_inv_f64_tile_kernel(
         for (int i = 0; i < 4; ++i) {
           const int p = (rg * RT + r) * 16 + i * 4 + kq;
           if (p < P2) {
-            __builtin_nontemporal_store(acc[0][r][i], out + geo.at(p, m, se, col));
-            __builtin_nontemporal_store(acc[1][r][i], out + geo.at(p, m, so, col));
+            put_four<NT>(out + geo.at(p, m, se, col), acc[0][r][i]);
+            put_four<NT>(out + geo.at(p, m, so, col), acc[1][r][i]);
           }
         }
     }
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void leg_inv_f64_tile_kernel(
       const int ln = el & 63, ri = el >> 6, i = ri & 3, r = ri >> 2;
       const int p = (rg * RT + r) * 16 + i * 4 + (ln >> 4);
       const int c = (ctg * cpw + g) * 16 + (ln & 15);
-      if (p < P2 && c < ncol) __builtin_nontemporal_store(v, out + geo.at(p, m, slot, c));
+      if (p < P2 && c < ncol) put_four<NT>(out + geo.at(p, m, slot, c), v);
     }
     __syncthreads();
   }
@@ -494,9 +494,10 @@ int inv_tile_launch(const Plan &p, KSplit ks, const double *spec, double *four, 
   const long nblk = ks.m2 == 0 ? xcd_m_blocks(p, 0, (int)w1)
                                : ks.m4 * w4 + (long)(ks.m2 - ks.m4) * w2 + (long)(p.N + 1 - ks.m2) * w1;
   const size_t lds = ks.m2 > 0 ? sizeof(double) * 4 * RT * 256 : 0;
-  launch_kernel(leg_inv_f64_tile_kernel<RT, D>, (int)nblk, 256, lds, s,
-      spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.zero, p.N, p.P2, p.P2p, ncol, nct, tab0,
-      ntab, ks, p.xcd_m[0], p.xcd_mlen[0], four_geom(p.N, p.im, nf));
+  const FourGeom geo = four_geom_out(p, nf, sizeof(double));
+  launch_kernel(geo.stream ? leg_inv_f64_tile_kernel<RT, D, true> : leg_inv_f64_tile_kernel<RT, D, false>,
+      (int)nblk, 256, lds, s, spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.zero, p.N, p.P2,
+      p.P2p, ncol, nct, tab0, ntab, ks, p.xcd_m[0], p.xcd_mlen[0], geo);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -506,9 +507,10 @@ int inv_wave_launch(const Plan &p, const double *spec, double *four, double *fou
                     int ntab, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int upm = ntab * (p.P2p / 16) * nct;
-  launch_kernel(leg_inv_f64_wave_kernel<D>, xcd_m_blocks(p, 0, (upm + 3) / 4), 256, 0, s,
-      spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p,
-      ncol, nct, tab0, ntab, four_geom(p.N, p.im, nf));
+  const FourGeom geo = four_geom_out(p, nf, sizeof(double));
+  launch_kernel(geo.stream ? leg_inv_f64_wave_kernel<D, true> : leg_inv_f64_wave_kernel<D, false>,
+      xcd_m_blocks(p, 0, (upm + 3) / 4), 256, 0, s, spec, four, four2, p.Pinv, p.Hinv, p.poff_inv,
+      p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, tab0, ntab, geo);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -669,9 +671,10 @@ int inv_launch(const Plan &p, const double *spec, double *four, double *four2, i
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int nblk = xcd_m_blocks(p, 0, ntab * nct);
   const size_t lds = sizeof(double) * kWaves * RT * 256;
-  launch_kernel(leg_inv_f64_kernel<RT>, nblk, 256, lds, s, spec, four, four2, p.Pinv, p.Hinv,
-                p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, tab0, ntab,
-                four_geom(p.N, p.im, nf));
+  const FourGeom geo = four_geom_out(p, nf, sizeof(double));
+  launch_kernel(geo.stream ? leg_inv_f64_kernel<RT, true> : leg_inv_f64_kernel<RT, false>, nblk, 256,
+                lds, s, spec, four, four2, p.Pinv, p.Hinv, p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N,
+                p.P2, p.P2p, ncol, nct, tab0, ntab, geo);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }

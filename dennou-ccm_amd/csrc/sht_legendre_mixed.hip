@@ -270,7 +270,7 @@ constexpr int kOutPitch = 36;
 // once, and the table fragments fetched per column.  Results leave through a
 // per-wave LDS transpose as 16-byte stores of 4 consecutive columns (the
 // Fourier layout keeps a field group's 2G columns contiguous).
-template <int MODE, int NSL, int CT>
+template <int MODE, int NSL, int CT, bool NT>
 __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
@@ -355,8 +355,8 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
         for (int h = 0; h < 2; ++h) {
           const int q = lane + 64 * h, pl = q >> 3, par = (q >> 2) & 1, c4 = (q & 3) * 4;
           const int p = rt * 16 + pl, cc = (ct0 + j) * 16 + c4;
-          const float4 v = *reinterpret_cast<const float4 *>(ob + pl * kOutPitch + par * 16 + c4);
-          if (p < P2 && cc < cend) *reinterpret_cast<float4 *>(four + geo.at(p, m, par, cc)) = v;
+          const f4 v = *reinterpret_cast<const f4 *>(ob + pl * kOutPitch + par * 16 + c4);
+          if (p < P2 && cc < cend) put_four<NT>(reinterpret_cast<f4 *>(four + geo.at(p, m, par, cc)), v);
         }
       } else if (col < ncol) {  // field groups of fewer than 4 columns (im = 2048)
 #pragma unroll
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int p = rt * 16 + kq * 4 + i;
-            if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[par][i];
+            if (p < P2) put_four<NT>(four + geo.at(p, m, par, col), (float)acc[par][i]);
           }
       }
     }
@@ -441,7 +441,7 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = rt * 16 + kq * 4 + i;
-          if (k < kc) sj[(size_t)(2 * k + par) * ncol] = acc[i];
+          if (k < kc) __builtin_nontemporal_store(acc[i], sj + (size_t)(2 * k + par) * ncol);
         }
       }
     }
@@ -481,7 +481,7 @@ __device__ __forceinline__ int col_exponent(const T (&v)[8]) {
   return mx > 0 ? 13 - ilogb(mx) : 0;
 }
 
-template <int MODE, int NSL>
+template <int MODE, int NSL, bool NT>
 __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
@@ -545,7 +545,7 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int p = rt * 16 + kq * 4 + i;
-          if (p < P2) four[geo.at(p, m, par, col)] = (float)acc[par][i];
+          if (p < P2) put_four<NT>(four + geo.at(p, m, par, col), (float)acc[par][i]);
         }
     }
   }
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_wave_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = kt * 16 + kq * 4 + i;
-          if (k < kc) sm[(size_t)(2 * k + par) * ncol] = acc[i];
+          if (k < kc) __builtin_nontemporal_store(acc[i], sm + (size_t)(2 * k + par) * ncol);
         }
       }
     }
@@ -711,20 +711,23 @@ int build_mixed_mode(Plan &p) {
 template <int MODE, int NSL>
 int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const FourGeom geo = four_geom_out(p, nf, sizeof(float));
   if (shape == SHT_SHAPE_WAVE) {
-    launch_kernel(leg_inv_mixed_wave_kernel<MODE, NSL>, xcd_m_blocks(p, 0, (nct + 3) / 4), 256, 0, s,
-                  spec, four, static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv,
-                  p.xcd_m[0], p.xcd_mlen[0], p.zero, p.N, p.P2, p.P2p, ncol, nct,
-                  four_geom(p.N, p.im, nf));
+    launch_kernel(geo.stream ? leg_inv_mixed_wave_kernel<MODE, NSL, true> : leg_inv_mixed_wave_kernel<MODE, NSL, false>,
+                  xcd_m_blocks(p, 0, (nct + 3) / 4), 256, 0, s, spec, four,
+                  static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
+                  p.xcd_mlen[0], p.zero, p.N, p.P2, p.P2p, ncol, nct, geo);
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }
   // KSPLIT: one column tile per workgroup; TILE: two
-  auto kern = shape == SHT_SHAPE_TILE ? leg_inv_mixed_kernel<MODE, NSL, 2> : leg_inv_mixed_kernel<MODE, NSL, 1>;
+  auto kern = shape == SHT_SHAPE_TILE
+                  ? (geo.stream ? leg_inv_mixed_kernel<MODE, NSL, 2, true> : leg_inv_mixed_kernel<MODE, NSL, 2, false>)
+                  : (geo.stream ? leg_inv_mixed_kernel<MODE, NSL, 1, true> : leg_inv_mixed_kernel<MODE, NSL, 1, false>);
   const int ct = shape == SHT_SHAPE_TILE ? 2 : 1;
   launch_kernel(kern, xcd_m_blocks(p, 0, (nct + ct - 1) / ct), 256, 0, s, spec, four,
                 static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
-                p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, four_geom(p.N, p.im, nf));
+                p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, geo);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
