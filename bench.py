@@ -244,7 +244,7 @@ def legendre_kernel_name(stage: str, fp64: bool, shape: int) -> str:
     if fp64:
         suffix = ("_f64_kernel", "_f64_wave_kernel", "_f64_tile_kernel", "_f64_tile_kernel")[shape]
     else:
-        suffix = ("_mixed_kernel", "_mixed_wave_kernel")[shape]
+        suffix = ("_mixed_kernel", "_mixed_wave_kernel", "_mixed_kernel")[shape]  # TILE: 2 column tiles
     return stage + suffix
 
 ENSEMBLE_GRIDS = {42: (128, 64), 85: (256, 128), 170: (512, 256)}
