@@ -373,8 +373,10 @@ def main():
     achieved = bytes_ / (kms[dom] * 1e-3) / 1e9
     hshapes = results[headline]["shapes"]
     if dom in KERNEL_NAMES:
-        kname = f"{KERNEL_NAMES[dom]}<{IM},{'double' if prec == 0 else 'float'}>"
-        traffic = traffic_from_profile(headline, kname)
+        # the forward FFT carries a third template argument (direct first pass)
+        kprefix = f"{KERNEL_NAMES[dom]}<{IM},{'double' if prec == 0 else 'float'}"
+        kname = kprefix + ">"
+        traffic = traffic_from_profile(headline, kprefix)
     else:
         kname = legendre_kernel_name(dom, prec == 0, hshapes[0 if dom == "leg_inv" else 1])
         traffic = traffic_from_profile(headline, kname + "<")
