@@ -25,6 +25,13 @@
 // feeds its first pass straight from the grid.
 #include "sht_internal.h"
 
+// No implicit fused multiply-adds in this file: the compiler picks which
+// product of a*b - c*d to fuse per instantiation, so the direct and
+// LDS-staged forward kernels (the same arithmetic) would round differently
+// and a field's spectrum would depend on the batch size.  The complex
+// multiply fuses explicitly instead (same operation count, fixed rounding).
+#pragma clang fp contract(off)
+
 namespace sht {
 namespace {
 
@@ -48,7 +55,7 @@ __device__ __forceinline__ void store_pair(float *o, double2 v) {
 }
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_double2(__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
