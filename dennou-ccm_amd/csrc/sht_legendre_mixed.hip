@@ -5,7 +5,9 @@
 //   SPLIT_BF16    x ~ hi + lo               (bf16, |x - hi - lo| <= 2^-18 |x|)
 //   SPLIT_BF16X6  x ~ hi + mid + lo         (bf16, <= 2^-27 |x|)
 //   SCALED_FP16   x 2^e ~ hi + lo 2^-11     (fp16, <= 2^-22 |x 2^e|)
-// and a product A.B is formed from the piece products whose magnitude is not
+// (bounds for the tables; the data operands are rounded to FP32 before the
+// split, adding up to 2^-25 |x|), and a product A.B is formed from the piece
+// products whose magnitude is not
 // below the split residual: hi.hi + hi.lo + lo.hi (three products) for the
 // two-piece forms, hi.hi + hi.mid + mid.hi + hi.lo + lo.hi + mid.mid (six) for
 // BF16X6.  Each v_mfma covers a 32-wide slice of the GEMM's K dimension and
@@ -84,11 +86,23 @@ __device__ __forceinline__ void split(double x, uint16_t (&pc)[3]) {
   }
 }
 
-// The same pieces for an FP32 value (the Fourier data of the forward
-// transform), in FP32 arithmetic: x has 24 significant bits, so every
-// residual is exact in FP32 -- the pieces equal those the FP64 split above
-// gives for (double)x -- and hi + mid + lo = x exactly for BF16X6.
-__device__ __forceinline__ uint16_t bf16_bits(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
+// The same pieces for an FP32 value, in FP32 arithmetic: x has 24
+// significant bits, so every residual is exact in FP32 -- the pieces equal
+// those the FP64 split above gives for (double)x -- and hi + mid + lo = x
+// exactly for BF16X6.  Both data operands are split this way: the forward's
+// Fourier data are FP32, and the inverse rounds each FP64 coefficient to
+// FP32 first, which keeps the bound of the three-piece form (2^-25 |x| from
+// the rounding, none from the split) at a fraction of the FP64 split's cost.
+// The FP64 split above serves the tables.
+// Round to nearest even on the FP32 bits (finite x).  Written with integer
+// operations on purpose: given (__bf16)(float)d for an FP64 d, the compiler
+// fuses the two roundings into one double -> bf16 rounding, which differs
+// from rounding the FP32 value when that value is a bf16 tie, and it did so
+// in one kernel shape but not in another.
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+  const uint32_t b = __float_as_uint(x);
+  return (uint16_t)((b + 0x7fffu + ((b >> 16) & 1u)) >> 16);
+}
 __device__ __forceinline__ float bf16_valf(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 template <int MODE>
 __device__ __forceinline__ void split(float x, uint16_t (&pc)[3]) {
@@ -318,7 +332,7 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
     const int kk = s * 32 + k, cc = j * 16 + c;
     const bool v = kk < (par ? ko : ke) && ct0 * 16 + cc < ncol;
     const double x = sm[(size_t)(v ? 2 * kk + par : 0) * ncol + (v ? cc : 0)];
-    return v ? x : 0.0;
+    return v ? (float)x : 0.0f;  // split as FP32: see split(float)
   });
   __syncthreads();
   for (int rt = wave; rt < rtiles; rt += 4) {
@@ -506,12 +520,12 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
   for (int q = 0; q < 2; ++q)
 #pragma unroll
     for (int sl = 0; sl < NSL; ++sl) {
-      double v[8];
+      float v[8];
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         const int k = sl * 32 + kq * 8 + i;
         const bool ok = sl < nsl && cok && k < (q ? ko : ke);
-        v[i] = *(ok ? sm + (size_t)(2 * k + q) * ncol : zero);
+        v[i] = (float)*(ok ? sm + (size_t)(2 * k + q) * ncol : zero);
       }
       const int e = col_exponent<MODE>(v);
       bsc[q][sl] = ldexp(1.0, -e);
