@@ -248,3 +248,28 @@ def test_error_codes():
         p.spec_to_grad(s, None, None, g, 2)  # GradLat without SHT_FLAG_GRAD
     assert e.value.code == shtamd.SHT_ERR_INVALID
     p.close()
+
+
+@pytest.mark.parametrize("im", [1024, 2048])
+@pytest.mark.parametrize("prec,tol", [(shtamd.PREC_FP64, TOL), (shtamd.PREC_SPLIT_BF16X6, 1e-6)])
+def test_long_rings(im, prec, tol):
+    """Ring lengths 1024 and 2048 (three FFT passes, G = 2 and 1 fields per
+    FFT block); at 2048 a field group is narrower than the 4 columns the mixed
+    inverse stores at once, so it takes the column-by-column store path.
+    Gradients too (the derivative table and the GradLon/GradLat FFT modes)."""
+    N, jm, nf = 42, 64, 7
+    say("long rings", im, prec)
+    spec = synth.spectra(N, nf, cfg=16)
+    spec[0] = np.arange(1.0, nf + 1.0)
+    ref = oracle.Plan(N, im, jm)
+    p = shtamd.Plan(N, im, jm, nf, precision=prec, flags=shtamd.FLAG_GRAD)
+    g = gpu_inverse(p, spec, nf)
+    assert rel(g, ref.spec_to_grid(spec)) < tol
+    assert rel(gpu_forward(p, g, nf), ref.grid_to_spec(g)) < 2 * tol
+    s = dev(shtamd.spec_to_real(spec))
+    outs = [torch.empty((nf, jm, im), dtype=torch.float64, device="cuda") for _ in range(3)]
+    p.spec_to_grad(s, *outs, nf)
+    torch.cuda.synchronize()
+    for kind, o in enumerate(outs):
+        assert rel(o.cpu().numpy(), ref.spec_to_grid(spec, kind=kind)) < tol
+    p.close()
