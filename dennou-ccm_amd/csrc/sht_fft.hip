@@ -14,7 +14,7 @@
 // The Fourier coefficients are read/written in the [pair][m][parity][field]
 // layout the Legendre GEMMs want, so this kernel is also the (lat,lon) <->
 // (m,lat) transpose: global traffic is row-contiguous on the grid side and
-// 2G-double runs on the Fourier side; the reordering happens in LDS.
+// runs of 2G values on the Fourier side; the reordering happens in LDS.
 // The complex FFT is a Stockham autosort with radix 8/16 butterflies held in
 // registers (two or three passes per transform) over a single 32 KB LDS
 // buffer holding G*L = 2048 complex doubles.  LDS bandwidth bounds the
@@ -59,12 +59,6 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-// multiply by SIGN * i
-template <int SIGN>
-__device__ __forceinline__ double2 mul_si(double2 a) {
-  return SIGN > 0 ? make_double2(-a.y, a.x) : make_double2(a.y, -a.x);
-}
-
 // e^{+2 pi i k/16}, k = 0..15
 __constant__ double kW16[16][2] = {
     {1.0, 0.0}, {0.92387953251128674, 0.38268343236508977}, {0.70710678118654752, 0.70710678118654752},

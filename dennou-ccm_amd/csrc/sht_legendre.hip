@@ -52,7 +52,6 @@ __global__ __launch_bounds__(256) void leg_inv_f64_kernel(
   if (!xcd_m_block(blockIdx.x, mlist, mlen, ntab * nct, m, t)) return;
   const int ct = t % nct;
   const int tab = tab0 + t / nct;
-  const int M1 = N + 1;
   const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP = (ke + 3) & ~3, nsteps = KP >> 2;
@@ -130,7 +129,6 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
   extern __shared__ double red[]; // [kWaves][MAXRT][4][64]
   int m, ct;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, nct, m, ct)) return;
-  const int M1 = N + 1;
   const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
@@ -223,7 +221,6 @@ __global__ __launch_bounds__(256) void leg_inv_f64_wave_kernel(
   const int rest = u / nct;
   const int rt = rest % rtiles;
   const int tab = tab0 + rest / rtiles;
-  const int M1 = N + 1;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP = (ke + 3) & ~3, nsteps = KP >> 2;
   const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m] + rt * 16 + r16;
@@ -290,7 +287,6 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_wave_kernel(
   const int ct = u % nct;
   const int mk = units[u / nct];
   const int m = mk >> 8, kt = mk & 255;
-  const int M1 = N + 1;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KR = (ke + 15) & ~15;
   const int nsteps = P2p >> 2;
@@ -365,7 +361,6 @@ __global__ __launch_bounds__(256) void leg_inv_f64_tile_kernel(
     int ncol, int nct, int tab0, int ntab, KSplit kspl, const int *__restrict__ mlist, int mlen,
     FourGeom geo) {
   extern __shared__ double red[];  // [4 waves][RT][4][64], one parity at a time
-  const int M1 = N + 1;
   const int ngr = (P2p >> 4) / RT;  // row groups
   const int w4 = ngr * ntab * nct, w2 = ngr * ntab * ((nct + 1) >> 1), w1 = ngr * ntab * ((nct + 3) >> 2);
   int b = blockIdx.x;
@@ -525,7 +520,6 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_tile_kernel(
     const int *__restrict__ poff, const double *__restrict__ zero, int N, int P2, int P2p,
     int ncol, int nct, KSplit kspl, const int *__restrict__ mlist, int mlen, FourGeom geo) {
   extern __shared__ double red[];  // [4 waves][MAXRT][4][64], one parity at a time
-  const int M1 = N + 1;
   const int w4 = nct, w2 = (nct + 1) >> 1, w1 = (nct + 3) >> 2;
   int b = blockIdx.x;
   int ks, m, cpw;

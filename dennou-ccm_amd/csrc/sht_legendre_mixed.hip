@@ -296,7 +296,6 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
   int m, t;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
   const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
-  const int M1 = N + 1;
   const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
@@ -401,7 +400,6 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
   int m, t;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
   const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
-  const int M1 = N + 1;
   const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
@@ -507,7 +505,6 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + 3) >> 2, m, t)) return;
   const int ct = t * 4 + wave;
   if (ct >= nct) return;
-  const int M1 = N + 1;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
   const int col = ct * 16 + r16;
@@ -577,7 +574,6 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_wave_kernel(
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + 3) >> 2, m, t)) return;
   const int ct = t * 4 + wave;
   if (ct >= nct) return;
-  const int M1 = N + 1;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
   const int nsl = P2p32 >> 5;
@@ -662,7 +658,7 @@ __global__ __launch_bounds__(256) void split_table_kernel(const double *__restri
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) {

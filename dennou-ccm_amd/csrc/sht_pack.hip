@@ -70,7 +70,7 @@ __global__ __launch_bounds__(256) void avr_kernel(const double *__restrict__ gri
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
   if (threadIdx.x == 0) avg[f] = red[0] / (2.0 * im);
