@@ -284,8 +284,14 @@ constexpr int kOutPitch = 36;
 // once, and the table fragments fetched per column.  Results leave through a
 // per-wave LDS transpose as 16-byte stores of 4 consecutive columns (the
 // Fourier layout keeps a field group's 2G columns contiguous).
+// One column tile per workgroup with at most two K slices (N <= 127) asks
+// the compiler for 5 waves per SIMD, so a T85 batch of 97 fields (1144
+// workgroups) is resident at once (measured at T85: neutral at 97 fields, 6 %
+// faster at 388; the six-product form spills two registers for it).  With
+// more slices the hint would spill heavily.
 template <int MODE, int NSL, int CT, bool NT>
-__global__ __launch_bounds__(256) void leg_inv_mixed_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((CT == 1 && NSL <= 2) ? 5 : 1))) void
+leg_inv_mixed_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
     int mlen, int N, int P2, int P2p, int ncol, int nct, FourGeom geo) {
