@@ -227,7 +227,8 @@ int sht_server_stats(const sht_server *server, long long *requests, long long *f
  *      recv(r_i, d) += send(s_i, d) * coef_i for every triplet i, in triplet
  *      order.  Indices are 0-based here (the reference's are 1-based).  The
  *      GPU sums each receiving point's triplets in their original order with
- *      unfused multiply and add, so results are bitwise those of the loop. */
+ *      unfused multiply and add, so results are bitwise those of the loop
+ *      evaluated without fused multiply-adds. */
 typedef struct sht_remap sht_remap;
 int sht_remap_create(const int *send_index, const int *recv_index, const double *coef, int nnz,
                      int nsend, int nrecv, sht_remap **remap);
