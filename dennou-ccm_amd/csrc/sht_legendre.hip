@@ -711,7 +711,12 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
     const int rtiles = p.P2p / 16;
     int rt = 4;
     while (rtiles % rt) --rt;
-    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_inv(p, 6, 3);
+    // TILE_SPLIT: halve the K loop of the wavenumbers with at least 70 % of the
+    // heaviest one's K steps (the waves that set the critical path); no
+    // four-way split.  Measured at T85: 97 fields 10.9 us (thresholds 6/3 gave
+    // 15.4, TILE 13.5); T42, 388 fields 7.2 us (9.1, TILE 12.1).
+    const int nsmax = roundup(ceil_half(p.N + 1), 4) / 4;
+    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_inv(p, 1 << 30, (7 * nsmax + 9) / 10);
     switch (rt) {
       case 4: return inv_tile_launch<4, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
       case 3: return inv_tile_launch<3, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
@@ -739,7 +744,10 @@ int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, doubl
   if (shape == SHT_SHAPE_WAVE) return fwd_wave_launch<4>(p, four, spec, nf, s);
   const int rt = (ceil_half(p.N + 1) + 15) / 16;
   if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
-    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_fwd(p, 2, 1);
+    // TILE_SPLIT: halve the pair loop of the wavenumbers with the most row
+    // tiles only (thresholds 2/1, which split every m, were slower everywhere)
+    const int nrtmax = roundup(ceil_half(p.N + 1), 16) / 16;
+    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_fwd(p, 1 << 30, nrtmax);
     switch (rt) {
       case 1: return fwd_tile_launch<1, 2>(p, k, four, spec, nf, s);
       case 2: return fwd_tile_launch<2, 2>(p, k, four, spec, nf, s);

@@ -264,7 +264,7 @@ enum {
   SHT_SHAPE_KSPLIT = 0,       /* workgroup per (m, 16 columns); its 4 waves split K     */
   SHT_SHAPE_WAVE = 1,         /* wavefront per 16x16 tile, whole K, pipelined loads     */
   SHT_SHAPE_TILE = 2,         /* wavefront per 16 columns x all row tiles (registers)   */
-  SHT_SHAPE_TILE_SPLIT = 3,   /* TILE with K split 4 or 2 ways for the heavy low m      */
+  SHT_SHAPE_TILE_SPLIT = 3,   /* TILE with the K loop halved for the heaviest m         */
   SHT_NSHAPE = 4
 };
 /* Fixes the shapes for calls with nf fields (nf = 0: the plan default for every
