@@ -110,7 +110,7 @@ class Plan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and lib is not None:  # module globals may be gone at interpreter exit
             lib().orc_plan_destroy(h)
             self._h = None
 
