@@ -235,14 +235,15 @@ def traffic_from_profile(form: str, kernel_prefix: str):
 
 
 KERNEL_NAMES = {"fft_inv": "fft_inv_kernel", "fft_fwd": "fft_fwd_kernel"}
-SHAPE_NAMES = ("ksplit", "wave", "tile", "tile_split")
+SHAPE_NAMES = ("ksplit", "wave", "tile", "tile_split", "staged")
 
 
 def legendre_kernel_name(stage: str, fp64: bool, shape: int) -> str:
     """Kernel name (as rocprof prints it, up to the template list) of a
     Legendre stage in a given shape."""
     if fp64:
-        suffix = ("_f64_kernel", "_f64_wave_kernel", "_f64_tile_kernel", "_f64_tile_kernel")[shape]
+        suffix = ("_f64_kernel", "_f64_wave_kernel", "_f64_tile_kernel", "_f64_tile_kernel",
+                  "_f64_staged_kernel")[shape]
     else:
         suffix = ("_mixed_kernel", "_mixed_wave_kernel", "_mixed_kernel")[shape]  # TILE: 2 column tiles
     return stage + suffix

@@ -659,6 +659,192 @@ int fwd_wave_launch(const Plan &p, const double *four, double *spec, int nf, hip
   return SHT_OK;
 }
 
+// Staged shape: the workgroup copies the data block of its (m, CT column
+// tiles) into LDS once -- for the forward all latitude pairs and both
+// parities of the Fourier data, for the inverse all degrees of the spectra --
+// and wave w then takes the (parity, 16-row tile) units w, w+4, ... and runs
+// each over the whole K loop: table fragments from global memory (one per
+// step, used for all CT column tiles), data fragments from LDS.  No partial
+// sums cross waves, and a wave holds only CT accumulators and its table
+// pipeline, so occupancy is set by the LDS block alone.  Built for the large
+// truncations, where the other shapes re-read the data tile per row tile or
+// hold every row tile's accumulators at once.  One column tile per
+// workgroup: two (table fragments used twice, twice the LDS) measured slower
+// at T42, T85 and T170; so did table loads 8 or 16 steps ahead instead of 4,
+// two accumulator chains per unit, and staging rows parity-fastest.
+//
+// Copies rows of 16 columns: element e of the block is the column pair c2 =
+// e % 8 of row r = (e / 8) % nrow of column tile j = e / (8 nrow); src(j, r,
+// col) returns columns col, col + 1 (col even) of the row, zeros outside the
+// data.  Loads are issued NQ at a time before their LDS stores.
+template <int NQ, class Src>
+__device__ __forceinline__ void stage_rows(double *sb, int nrow, int nj, Src src) {
+  const int nel = nj * nrow * 8;
+  for (int e0 = threadIdx.x; e0 < nel; e0 += NQ * 256) {
+    double2 v[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = e0 + q * 256, c2 = e & 7, r = (e >> 3) % nrow, j = (e >> 3) / nrow;
+      v[q] = e < nel ? src(j, r, 2 * c2) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int e = e0 + q * 256;
+      if (e < nel) *reinterpret_cast<double2 *>(sb + 2 * e) = v[q];
+    }
+  }
+}
+
+// Runs one unit: acc[j] += A(step) B_j(step) over nsteps steps of 4 K values,
+// A from global (a(s) = the lane's table value of step s, any s >= 0 valid),
+// B from LDS (b(j, s)).  Table loads run D steps ahead.
+template <int CT, int D, class ALoad, class BLoad>
+__device__ __forceinline__ void staged_unit(int nsteps, ALoad a_at, BLoad b_at, d4 (&acc)[CT]) {
+  double a[2][D];
+  auto load = [&](int buf, int s0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) a[buf][j] = a_at(s0 + j < nsteps ? s0 + j : 0);
+  };
+  auto run = [&](int buf, int s0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      if (s0 + j < nsteps)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) acc[c] = mfma_f64(a[buf][j], b_at(c, s0 + j), acc[c]);
+  };
+  load(0, 0);
+  for (int s0 = 0; s0 < nsteps; s0 += 2 * D) {
+    load(1, s0 + D);
+    run(0, s0);
+    if (s0 + D >= nsteps) break;
+    load(0, s0 + 2 * D);
+    run(1, s0 + D);
+  }
+}
+
+// Forward, workgroup = (m, CT column tiles); LDS [CT][parity][P2p pairs][16].
+template <int CT, int D>
+__global__ __launch_bounds__(256) void leg_fwd_f64_staged_kernel(
+    const double *__restrict__ four, double *__restrict__ spec, const double *__restrict__ Pfwd,
+    const int *__restrict__ poff, const int *__restrict__ mlist, int mlen, int N, int P2, int P2p,
+    int ncol, int nct, FourGeom geo) {
+  extern __shared__ double sb[];
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
+  const int ct0 = t * CT;
+  // the Fourier scratch is 16-byte aligned and its column runs are even
+  stage_rows<4>(sb, 2 * P2p, CT, [&](int j, int r, int c) {
+    const int par = r / P2p, p = r - par * P2p, col = (ct0 + j) * 16 + c;
+    return (p < P2 && col < ncol) ? *reinterpret_cast<const double2 *>(four + geo.at(p, m, par, col))
+                                  : make_double2(0.0, 0.0);
+  });
+  __syncthreads();
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KR = (ke + 15) & ~15, nrt = KR >> 4;
+  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol;
+  for (int u = wave; u < 2 * nrt; u += 4) {
+    const int q = u & 1, rt = u >> 1;
+    const double *__restrict__ W = Pfwd + poff[m] + (size_t)q * P2p * KR + rt * 16 + r16;
+    const double *B = sb + (q * P2p + kq) * 16 + r16;
+    d4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+    staged_unit<CT, D>(
+        P2p >> 2, [&](int s) { return W[(size_t)(s * 4 + kq) * KR]; },
+        [&](int c, int s) { return B[(c * 2 * P2p + s * 4) * 16]; }, acc);
+    const int kn = q ? ko : ke;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int col = (ct0 + c) * 16 + r16;
+      if (col < ncol)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = rt * 16 + i * 4 + kq;
+          if (k < kn) __builtin_nontemporal_store(acc[c][i], sm + (size_t)(2 * k + q) * ncol + col);
+        }
+    }
+  }
+}
+
+// Inverse, workgroup = (m, table, CT column tiles); LDS [CT][parity][KP][16]
+// with KP = this m's degrees per parity rounded up to 4 (rows past a parity's
+// last degree hold zeros).
+template <int CT, int D, bool NT>
+__global__ __launch_bounds__(256) void leg_inv_f64_staged_kernel(
+    const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
+    const double *__restrict__ Pinv, const double *__restrict__ Hinv, const int *__restrict__ poff,
+    const int *__restrict__ mlist, int mlen, int N, int P2, int P2p, int ncol, int nct, int tab0,
+    int ntab, FourGeom geo) {
+  extern __shared__ double sb[];
+  const int ncg = (nct + CT - 1) / CT;
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, ntab * ncg, m, t)) return;
+  const int tab = tab0 + t % ntab, ct0 = (t / ntab) * CT;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP = (ke + 3) & ~3;
+  const double *__restrict__ sp = spec + (size_t)coef_off(N, m) * ncol;
+  // the caller's spectra may be only 8-byte aligned: two loads per pair
+  stage_rows<4>(sb, 2 * KP, CT, [&](int j, int r, int c) {
+    const int par = r / KP, k = r - par * KP, col = (ct0 + j) * 16 + c;
+    if (k >= (par ? ko : ke) || col >= ncol) return make_double2(0.0, 0.0);
+    const double *x = sp + (size_t)(2 * k + par) * ncol + col;
+    return make_double2(x[0], x[1]);
+  });
+  __syncthreads();
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int rtiles = P2p >> 4;
+  double *__restrict__ out = tab ? four2 : four;
+  const int se = tab ? 1 : 0;  // the derivative table's parities are swapped
+  for (int u = wave; u < 2 * rtiles; u += 4) {
+    const int q = u & 1, rt = u >> 1;
+    const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m] + (size_t)q * KP * P2p + rt * 16 + r16;
+    const double *B = sb + (q * KP + kq) * 16 + r16;
+    d4 acc[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[c] = d4{0.0, 0.0, 0.0, 0.0};
+    staged_unit<CT, D>(
+        KP >> 2, [&](int s) { return T[(size_t)(s * 4 + kq) * P2p]; },
+        [&](int c, int s) { return B[(c * 2 * KP + s * 4) * 16]; }, acc);
+    const int slot = q ^ se;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int col = (ct0 + c) * 16 + r16;
+      if (col < ncol)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int p = rt * 16 + i * 4 + kq;
+          if (p < P2) put_four<NT>(out + geo.at(p, m, slot, col), acc[c][i]);
+        }
+    }
+  }
+}
+
+template <int CT>
+int fwd_staged_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const size_t lds = sizeof(double) * CT * 2 * p.P2p * 16;
+  launch_kernel(leg_fwd_f64_staged_kernel<CT, 4>, xcd_m_blocks(p, 1, (nct + CT - 1) / CT), 256, lds, s,
+      four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
+      four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
+template <int CT>
+int inv_staged_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0,
+                      int ntab, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const int kp0 = roundup(ceil_half(p.N + 1), 4);
+  const size_t lds = sizeof(double) * CT * 2 * kp0 * 16;
+  const FourGeom geo = four_geom_out(p, nf, sizeof(double));
+  launch_kernel(geo.stream ? leg_inv_f64_staged_kernel<CT, 4, true> : leg_inv_f64_staged_kernel<CT, 4, false>,
+      xcd_m_blocks(p, 0, ntab * ((nct + CT - 1) / CT)), 256, lds, s, spec, four, four2, p.Pinv, p.Hinv,
+      p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, tab0, ntab, geo);
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 template <int RT>
 int inv_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0, int ntab,
                int nf, hipStream_t s) {
@@ -694,10 +880,13 @@ int resolve_shape(const Plan &p, int nf, int dir) {
   const int d = dir ? p.shape_fwd : p.shape_inv;
   if (d != SHT_SHAPE_AUTO) return d;
   if (p.prec != SHT_PREC_FP64) return SHT_SHAPE_KSPLIT;
-  // measured at T85 on MI355X: register-blocked tiles win the inverse once
-  // there are enough column tiles to fill the chip; the K split wins the forward
-  if (dir == 1) return SHT_SHAPE_KSPLIT;
-  return nf >= 32 ? SHT_SHAPE_TILE : SHT_SHAPE_WAVE;
+  // measured on MI355X from T21 to T170 and 16 to 388 fields: the staged
+  // shape wins the forward nearly everywhere and the inverse from 32 fields;
+  // below that a wavefront per tile fills the chip better.  STAGED and WAVE
+  // run the same MFMA sequence per output tile, so the switch with the field
+  // count changes no result.
+  if (dir == 1) return SHT_SHAPE_STAGED;
+  return nf >= 32 ? SHT_SHAPE_STAGED : SHT_SHAPE_WAVE;
 }
 
 int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, double *four,
@@ -707,6 +896,7 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
   const int tab0 = value_table ? 0 : 1;
   const int ntab = (value_table && deriv_table) ? 2 : 1;
   if (shape == SHT_SHAPE_WAVE) return inv_wave_launch<4>(p, spec, four, four2, tab0, ntab, nf, s);
+  if (shape == SHT_SHAPE_STAGED) return inv_staged_launch<1>(p, spec, four, four2, tab0, ntab, nf, s);
   if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
     const int rtiles = p.P2p / 16;
     int rt = 4;
@@ -742,6 +932,7 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
 int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
                              hipStream_t s) {
   if (shape == SHT_SHAPE_WAVE) return fwd_wave_launch<4>(p, four, spec, nf, s);
+  if (shape == SHT_SHAPE_STAGED) return fwd_staged_launch<1>(p, four, spec, nf, s);
   const int rt = (ceil_half(p.N + 1) + 15) / 16;
   if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
     // TILE_SPLIT: halve the pair loop of the wavenumbers with the most row

@@ -26,8 +26,8 @@ SHT_ERR_BUSY = -5
 
 PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16, PREC_SPLIT_BF16X6 = 0, 1, 2, 3
 DIR_INVERSE, DIR_FORWARD, DIR_ROUNDTRIP = 0, 1, 2
-SHAPE_AUTO, SHAPE_KSPLIT, SHAPE_WAVE, SHAPE_TILE, SHAPE_TILE_SPLIT = -1, 0, 1, 2, 3
-SHAPES = ("ksplit", "wave", "tile", "tile_split")
+SHAPE_AUTO, SHAPE_KSPLIT, SHAPE_WAVE, SHAPE_TILE, SHAPE_TILE_SPLIT, SHAPE_STAGED = -1, 0, 1, 2, 3, 4
+SHAPES = ("ksplit", "wave", "tile", "tile_split", "staged")
 NSHAPE = len(SHAPES)
 FLAG_GRAD = 1
 

@@ -265,7 +265,9 @@ enum {
   SHT_SHAPE_WAVE = 1,         /* wavefront per 16x16 tile, whole K, pipelined loads     */
   SHT_SHAPE_TILE = 2,         /* wavefront per 16 columns x all row tiles (registers)   */
   SHT_SHAPE_TILE_SPLIT = 3,   /* TILE with the K loop halved for the heaviest m         */
-  SHT_NSHAPE = 4
+  SHT_SHAPE_STAGED = 4,       /* workgroup per (m, 16 columns), data block in LDS, each
+                                 wave a (parity, row tile) over the whole K (FP64 only)  */
+  SHT_NSHAPE = 5
 };
 /* Fixes the shapes for calls with nf fields (nf = 0: the plan default for every
  * count without an entry).  FP64 plans take every shape.  Mixed-precision

@@ -125,19 +125,27 @@ def test_shape_results_do_not_depend_on_batch():
 
 
 @pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (42, 128, 64, 9), (170, 512, 256, 13)])
-def test_wave_and_tile_inverse_bitwise_equal(N, im, jm, nf):
-    """WAVE and TILE run the same MFMA sequence per output tile (steps in order,
-    padded steps adding exact zeros), so the default inverse shape may switch
-    between them with the field count without changing any result."""
-    say("wave vs tile", N, nf)
+def test_wave_tile_staged_bitwise_equal(N, im, jm, nf):
+    """WAVE, TILE and STAGED run the same MFMA sequence per output tile of the
+    inverse (steps in order, padded steps adding exact zeros), and WAVE and
+    STAGED per tile of the forward, so the default shapes may switch between
+    them with the field count without changing any result."""
+    say("wave vs tile vs staged", N, nf)
     spec = synth.spectra(N, nf, cfg=15)
     p = shtamd.Plan(N, im, jm, nf, flags=shtamd.FLAG_GRAD)
     out = []
-    for sh in (shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE):
+    for sh in (shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE, shtamd.SHAPE_STAGED):
         p.set_shapes(0, sh, shtamd.SHAPE_KSPLIT)
         out.append(run_inverse(p, spec, nf, grad=True))
-    for a, b in zip(*out):
-        assert np.array_equal(a, b)
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+    grid = out[0][0]
+    fwd = []
+    for sh in (shtamd.SHAPE_WAVE, shtamd.SHAPE_STAGED):
+        p.set_shapes(0, shtamd.SHAPE_KSPLIT, sh)
+        fwd.append(run_forward(p, grid, nf))
+    assert np.array_equal(fwd[0], fwd[1])
     p.close()
 
 
@@ -230,11 +238,13 @@ def test_size_dependent_paths_bitwise_equal(prec):
     say("size paths", prec)
     spec = synth.spectra(N, big, cfg=15)
     p = shtamd.Plan(N, im, jm, big, precision=prec)
-    p.set_shapes(0, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT)
-    g_big = run_inverse(p, spec, big)[0]
-    g_small = run_inverse(p, spec[:, :small], small)[0]
-    assert np.array_equal(g_big[:small], g_small)
-    s_big = run_forward(p, g_big, big)
-    s_small = run_forward(p, g_big[:small], small)
-    assert np.array_equal(s_big[:, :small], s_small)
+    shapes = [shtamd.SHAPE_KSPLIT] + ([shtamd.SHAPE_STAGED] if prec == shtamd.PREC_FP64 else [])
+    for sh in shapes:
+        p.set_shapes(0, sh, sh)
+        g_big = run_inverse(p, spec, big)[0]
+        g_small = run_inverse(p, spec[:, :small], small)[0]
+        assert np.array_equal(g_big[:small], g_small)
+        s_big = run_forward(p, g_big, big)
+        s_small = run_forward(p, g_big[:small], small)
+        assert np.array_equal(s_big[:, :small], s_small)
     p.close()
