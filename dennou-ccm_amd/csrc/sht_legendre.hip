@@ -695,9 +695,11 @@ __device__ __forceinline__ void stage_rows(double *sb, int nrow, int nj, Src src
   }
 }
 
-// Runs one unit: acc[j] += A(step) B_j(step) over nsteps steps of 4 K values,
-// A from global (a(s) = the lane's table value of step s, any s >= 0 valid),
-// B from LDS (b(j, s)).  Table loads run D steps ahead.
+// Runs one unit: acc[c] += A(s) B_c(s) over the steps s < nsteps of 4 K
+// values each, in order; a_at(s) loads the lane's table value of step s from
+// global memory (called with s < nsteps only: steps past the end load step
+// 0 and are not run), b_at(c, s) reads column tile c's data from LDS.  Table
+// loads run D steps ahead.
 template <int CT, int D, class ALoad, class BLoad>
 __device__ __forceinline__ void staged_unit(int nsteps, ALoad a_at, BLoad b_at, d4 (&acc)[CT]) {
   double a[2][D];

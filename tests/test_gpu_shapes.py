@@ -181,12 +181,16 @@ def test_shape_argument_checks():
         p.set_shapes(9, 0, 0)  # nf above kmax
     p.set_shapes(4, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_KSPLIT)
     assert p.get_shapes(4) == (shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_KSPLIT)
+    # built-in choice: staged forward; inverse staged from 32 fields, wave below
+    assert p.get_shapes(0) == (shtamd.SHAPE_WAVE, shtamd.SHAPE_STAGED)
     p.set_shapes(4, shtamd.SHAPE_AUTO, shtamd.SHAPE_AUTO)
     assert p.get_shapes(4) == p.get_shapes(0)
     p.close()
     m = shtamd.Plan(21, 64, 32, 8, precision=shtamd.PREC_SPLIT_BF16X6)
     with pytest.raises(shtamd.ShtError):
         m.set_shapes(0, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_TILE_SPLIT)
+    with pytest.raises(shtamd.ShtError):  # the staged shape is FP64 only
+        m.set_shapes(0, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_STAGED)
     m.set_shapes(0, shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     assert m.get_shapes(5) == (shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     (inv, fwd), _ = m.autotune(8, reps=2)
