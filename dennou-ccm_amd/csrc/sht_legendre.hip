@@ -671,7 +671,10 @@ int fwd_wave_launch(const Plan &p, const double *four, double *spec, int nf, hip
 // hold every row tile's accumulators at once.  One column tile per
 // workgroup: two (table fragments used twice, twice the LDS) measured slower
 // at T42, T85 and T170; so did table loads 8 or 16 steps ahead instead of 4,
-// two accumulator chains per unit, and staging rows parity-fastest.
+// two accumulator chains per unit, staging rows parity-fastest or in one
+// round of loads, and holding a wave's results until its last unit with the
+// LDS data of a chunk read ahead (T170 forward at 776 fields 320 against
+// 296 us).
 //
 // Copies rows of 16 columns: element e of the block is the column pair c2 =
 // e % 8 of row r = (e / 8) % nrow of column tile j = e / (8 nrow); src(j, r,
