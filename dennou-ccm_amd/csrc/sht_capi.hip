@@ -1,4 +1,5 @@
 // sht_capi.hip -- the extern "C" entry points of libsht_hip.so (include/sht.h).
+#include <cmath>
 #include <cstring>
 #include <new>
 
@@ -289,15 +290,18 @@ int sht_plan_stage_times(sht_plan *h, double *total_ms, long long *launches) {
 
 int sht_l_nm(int n, int m) {
   if (n < 0 || m < -n || m > n) return fail(SHT_ERR_INVALID, "sht_l_nm: need 0 <= |m| <= n");
+  // l <= (n + 1)^2 must fit an int
+  if (n > 46339) return fail(SHT_ERR_INVALID, "sht_l_nm: n above 46339 overflows l");
   return n * (n + 1) + m + 1;
 }
 
 int sht_nm_l(int l, int *n, int *m) {
   if (l < 1 || !n || !m) return fail(SHT_ERR_INVALID, "sht_nm_l: need l >= 1");
-  int nn = 0;
-  while ((nn + 1) * (nn + 1) < l) ++nn;
-  *n = nn;
-  *m = l - 1 - nn * (nn + 1);
+  long long nn = (long long)std::sqrt((double)(l - 1));  // n = floor(sqrt(l - 1)), corrected below
+  while (nn > 0 && nn * nn > (long long)l - 1) --nn;
+  while ((nn + 1) * (nn + 1) <= (long long)l - 1) ++nn;
+  *n = (int)nn;
+  *m = (int)((long long)l - 1 - nn * (nn + 1));
   return SHT_OK;
 }
 

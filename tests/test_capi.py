@@ -55,6 +55,12 @@ def test_spectral_index_helpers():
             assert L.sht_nm_l(l, ctypes.byref(n), ctypes.byref(m)) == 0
             assert (n.value, m.value) == (nn, mm)
     assert L.sht_l_nm(2, 3) == shtamd.SHT_ERR_INVALID
+    # extremes: the largest n whose (n + 1)^2 fits an int, and the largest l
+    assert L.sht_l_nm(46339, 46339) == 46340 ** 2
+    assert L.sht_l_nm(46340, 0) == shtamd.SHT_ERR_INVALID
+    for l in (46340 ** 2, 46340 ** 2 + 1, 2 ** 31 - 1):
+        assert L.sht_nm_l(l, ctypes.byref(n), ctypes.byref(m)) == 0
+        assert n.value * (n.value + 1) + m.value + 1 == l and abs(m.value) <= n.value
 
 
 def test_null_plan_calls_fail_cleanly():
