@@ -314,8 +314,11 @@ void sht_server_destroy(sht_server *S) {
     S->stop = true;
   }
   S->cv.notify_all();
+  // the worker returns once nothing is pending or in flight; the stream
+  // waits only cover a batch whose launch failed part-way
   if (S->worker.joinable()) S->worker.join();
-  (void)hipDeviceSynchronize();
+  for (auto &sl : S->slots)
+    if (sl.s) (void)hipStreamSynchronize(sl.s);
   server_release(S);
 }
 
