@@ -662,37 +662,45 @@ int fwd_wave_launch(const Plan &p, const double *four, double *spec, int nf, hip
 // Staged shape: the workgroup copies the data block of its (m, CT column
 // tiles) into LDS once -- for the forward all latitude pairs and both
 // parities of the Fourier data, for the inverse all degrees of the spectra --
-// and wave w then takes the (parity, 16-row tile) units w, w+4, ... and runs
+// and wave w then takes the (parity, 16-row tile) units w, w+NW, ... and runs
 // each over the whole K loop: table fragments from global memory (one per
 // step, used for all CT column tiles), data fragments from LDS.  No partial
 // sums cross waves, and a wave holds only CT accumulators and its table
 // pipeline, so occupancy is set by the LDS block alone.  Built for the large
 // truncations, where the other shapes re-read the data tile per row tile or
-// hold every row tile's accumulators at once.  One column tile per
-// workgroup: two (table fragments used twice, twice the LDS) measured slower
-// at T42, T85 and T170; so did table loads 8 or 16 steps ahead instead of 4,
-// two accumulator chains per unit, staging rows parity-fastest or in one
-// round of loads, and holding a wave's results until its last unit with the
-// LDS data of a chunk read ahead (T170 forward at 776 fields 320 against
-// 296 us).
+// hold every row tile's accumulators at once.  Per output tile the MFMA
+// sequence depends neither on NW nor on CT.
 //
-// Copies rows of 16 columns: element e of the block is the column pair c2 =
-// e % 8 of row r = (e / 8) % nrow of column tile j = e / (8 nrow); src(j, r,
-// col) returns columns col, col + 1 (col even) of the row, zeros outside the
-// data.  Loads are issued NQ at a time before their LDS stores.
-template <int NQ, class Src>
+// NW waves share the block: from P2p = 128 latitude pairs (the T170 grid)
+// eight, and the inverse there takes two column tiles (each table fragment
+// feeds two MFMAs, halving the table traffic from L2); below that four waves
+// and one column tile.  Measured at T170, 776 fields: inverse 248 against
+// 273 us with four waves and one tile, forward 277 against 295 us (two tiles
+// for the forward: 276 us, but slower at 97 and 388 fields); eight waves were
+// slower at T42 and T85.  Measured no faster: table loads 8 or 16 steps
+// ahead instead of 4, two accumulator chains per unit, staging rows
+// parity-fastest or in one round of loads; slower: holding a wave's results
+// until its last unit with a chunk's LDS data read ahead (T170 forward 320
+// against 296 us).
+//
+// Copies rows of 16 columns with NT threads: element e of the block is the
+// column pair c2 = e % 8 of row r = (e / 8) % nrow of column tile
+// j = e / (8 nrow); src(j, r, col) returns columns col, col + 1 (col even) of
+// the row, zeros outside the data.  Loads are issued NQ at a time before
+// their LDS stores.
+template <int NQ, int NT, class Src>
 __device__ __forceinline__ void stage_rows(double *sb, int nrow, int nj, Src src) {
   const int nel = nj * nrow * 8;
-  for (int e0 = threadIdx.x; e0 < nel; e0 += NQ * 256) {
+  for (int e0 = threadIdx.x; e0 < nel; e0 += NQ * NT) {
     double2 v[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int e = e0 + q * 256, c2 = e & 7, r = (e >> 3) % nrow, j = (e >> 3) / nrow;
+      const int e = e0 + q * NT, c2 = e & 7, r = (e >> 3) % nrow, j = (e >> 3) / nrow;
       v[q] = e < nel ? src(j, r, 2 * c2) : make_double2(0.0, 0.0);
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-      const int e = e0 + q * 256;
+      const int e = e0 + q * NT;
       if (e < nel) *reinterpret_cast<double2 *>(sb + 2 * e) = v[q];
     }
   }
@@ -728,8 +736,8 @@ __device__ __forceinline__ void staged_unit(int nsteps, ALoad a_at, BLoad b_at, 
 }
 
 // Forward, workgroup = (m, CT column tiles); LDS [CT][parity][P2p pairs][16].
-template <int CT, int D>
-__global__ __launch_bounds__(256) void leg_fwd_f64_staged_kernel(
+template <int CT, int D, int NW>
+__global__ __launch_bounds__(NW * 64) void leg_fwd_f64_staged_kernel(
     const double *__restrict__ four, double *__restrict__ spec, const double *__restrict__ Pfwd,
     const int *__restrict__ poff, const int *__restrict__ mlist, int mlen, int N, int P2, int P2p,
     int ncol, int nct, FourGeom geo) {
@@ -738,7 +746,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_staged_kernel(
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
   const int ct0 = t * CT;
   // the Fourier scratch is 16-byte aligned and its column runs are even
-  stage_rows<4>(sb, 2 * P2p, CT, [&](int j, int r, int c) {
+  stage_rows<4, NW * 64>(sb, 2 * P2p, CT, [&](int j, int r, int c) {
     const int par = r / P2p, p = r - par * P2p, col = (ct0 + j) * 16 + c;
     return (p < P2 && col < ncol) ? *reinterpret_cast<const double2 *>(four + geo.at(p, m, par, col))
                                   : make_double2(0.0, 0.0);
@@ -748,7 +756,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_staged_kernel(
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KR = (ke + 15) & ~15, nrt = KR >> 4;
   double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol;
-  for (int u = wave; u < 2 * nrt; u += 4) {
+  for (int u = wave; u < 2 * nrt; u += NW) {
     const int q = u & 1, rt = u >> 1;
     const double *__restrict__ W = Pfwd + poff[m] + (size_t)q * P2p * KR + rt * 16 + r16;
     const double *B = sb + (q * P2p + kq) * 16 + r16;
@@ -775,8 +783,8 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_staged_kernel(
 // Inverse, workgroup = (m, table, CT column tiles); LDS [CT][parity][KP][16]
 // with KP = this m's degrees per parity rounded up to 4 (rows past a parity's
 // last degree hold zeros).
-template <int CT, int D, bool NT>
-__global__ __launch_bounds__(256) void leg_inv_f64_staged_kernel(
+template <int CT, int D, int NW, bool NT>
+__global__ __launch_bounds__(NW * 64) void leg_inv_f64_staged_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
     const double *__restrict__ Pinv, const double *__restrict__ Hinv, const int *__restrict__ poff,
     const int *__restrict__ mlist, int mlen, int N, int P2, int P2p, int ncol, int nct, int tab0,
@@ -790,7 +798,7 @@ __global__ __launch_bounds__(256) void leg_inv_f64_staged_kernel(
   const int KP = (ke + 3) & ~3;
   const double *__restrict__ sp = spec + (size_t)coef_off(N, m) * ncol;
   // the caller's spectra may be only 8-byte aligned: two loads per pair
-  stage_rows<4>(sb, 2 * KP, CT, [&](int j, int r, int c) {
+  stage_rows<4, NW * 64>(sb, 2 * KP, CT, [&](int j, int r, int c) {
     const int par = r / KP, k = r - par * KP, col = (ct0 + j) * 16 + c;
     if (k >= (par ? ko : ke) || col >= ncol) return make_double2(0.0, 0.0);
     const double *x = sp + (size_t)(2 * k + par) * ncol + col;
@@ -801,7 +809,7 @@ __global__ __launch_bounds__(256) void leg_inv_f64_staged_kernel(
   const int rtiles = P2p >> 4;
   double *__restrict__ out = tab ? four2 : four;
   const int se = tab ? 1 : 0;  // the derivative table's parities are swapped
-  for (int u = wave; u < 2 * rtiles; u += 4) {
+  for (int u = wave; u < 2 * rtiles; u += NW) {
     const int q = u & 1, rt = u >> 1;
     const double *__restrict__ T = (tab ? Hinv : Pinv) + poff[m] + (size_t)q * KP * P2p + rt * 16 + r16;
     const double *B = sb + (q * KP + kq) * 16 + r16;
@@ -825,26 +833,27 @@ __global__ __launch_bounds__(256) void leg_inv_f64_staged_kernel(
   }
 }
 
-template <int CT>
+template <int CT, int NW>
 int fwd_staged_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
-  const size_t lds = sizeof(double) * CT * 2 * p.P2p * 16;
-  launch_kernel(leg_fwd_f64_staged_kernel<CT, 4>, xcd_m_blocks(p, 1, (nct + CT - 1) / CT), 256, lds, s,
-      four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
+  const size_t lds = sizeof(double) * CT * 2 * p.P2p * 16;  // <= 64 KB: P2p <= 256
+  launch_kernel(leg_fwd_f64_staged_kernel<CT, 4, NW>, xcd_m_blocks(p, 1, (nct + CT - 1) / CT), NW * 64, lds,
+      s, four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
       four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
 
-template <int CT>
+template <int CT, int NW>
 int inv_staged_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0,
                       int ntab, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
   const int kp0 = roundup(ceil_half(p.N + 1), 4);
-  const size_t lds = sizeof(double) * CT * 2 * kp0 * 16;
+  const size_t lds = sizeof(double) * CT * 2 * kp0 * 16;  // <= 64 KB: kp0 <= 128
   const FourGeom geo = four_geom_out(p, nf, sizeof(double));
-  launch_kernel(geo.stream ? leg_inv_f64_staged_kernel<CT, 4, true> : leg_inv_f64_staged_kernel<CT, 4, false>,
-      xcd_m_blocks(p, 0, ntab * ((nct + CT - 1) / CT)), 256, lds, s, spec, four, four2, p.Pinv, p.Hinv,
+  launch_kernel(geo.stream ? leg_inv_f64_staged_kernel<CT, 4, NW, true>
+                           : leg_inv_f64_staged_kernel<CT, 4, NW, false>,
+      xcd_m_blocks(p, 0, ntab * ((nct + CT - 1) / CT)), NW * 64, lds, s, spec, four, four2, p.Pinv, p.Hinv,
       p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, tab0, ntab, geo);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
@@ -901,7 +910,9 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
   const int tab0 = value_table ? 0 : 1;
   const int ntab = (value_table && deriv_table) ? 2 : 1;
   if (shape == SHT_SHAPE_WAVE) return inv_wave_launch<4>(p, spec, four, four2, tab0, ntab, nf, s);
-  if (shape == SHT_SHAPE_STAGED) return inv_staged_launch<1>(p, spec, four, four2, tab0, ntab, nf, s);
+  if (shape == SHT_SHAPE_STAGED)
+    return p.P2p >= 128 ? inv_staged_launch<2, 8>(p, spec, four, four2, tab0, ntab, nf, s)
+                        : inv_staged_launch<1, 4>(p, spec, four, four2, tab0, ntab, nf, s);
   if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
     const int rtiles = p.P2p / 16;
     int rt = 4;
@@ -937,7 +948,8 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
 int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
                              hipStream_t s) {
   if (shape == SHT_SHAPE_WAVE) return fwd_wave_launch<4>(p, four, spec, nf, s);
-  if (shape == SHT_SHAPE_STAGED) return fwd_staged_launch<1>(p, four, spec, nf, s);
+  if (shape == SHT_SHAPE_STAGED)
+    return p.P2p >= 128 ? fwd_staged_launch<1, 8>(p, four, spec, nf, s) : fwd_staged_launch<1, 4>(p, four, spec, nf, s);
   const int rt = (ceil_half(p.N + 1) + 15) / 16;
   if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
     // TILE_SPLIT: halve the pair loop of the wavenumbers with the most row
