@@ -198,7 +198,7 @@ __device__ __forceinline__ void slice_mac2(const uint4 (&pa0)[3], const uint4 (&
 // values padded to 40 (80 bytes): eight lanes reading the same K octet of
 // consecutive columns then hit eight different 16-byte bank groups.
 constexpr int kRowPitch = 40;
-// Every wave converts whole groups (wave w takes groups w, w+4, ...): lane l
+// Every wave converts whole groups (wave w of NW takes groups w, w+NW, ...): lane l
 // reads the 8 values K = 8 (l/16) .. +7 of column l % 16 (load(g, k, c) returns
 // the FP64 value or 0 outside the tile), the FP16 exponent comes from the
 // maximum over the group's 32 values (a 4-lane shuffle reduction), and the
@@ -207,23 +207,23 @@ constexpr int kRowPitch = 40;
 // converts any, so one memory latency covers them (groups past ngroups load
 // group 0 again and are not stored).  Group g lands in LDS slot slot(g), so a
 // caller can stage only the groups that hold data.
-template <int MODE, int NIT, class Slot, class Load>
+template <int MODE, int NIT, int NW, class Slot, class Load>
 __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Slot slot, Load load) {
   constexpr int NP = Form<MODE>::NP;
   const int lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
   using T = decltype(load(0, 0, 0));  // double (spectra) or float (Fourier data)
-  for (int g0 = wave_id(); g0 < ngroups; g0 += 4 * NIT) {
+  for (int g0 = wave_id(); g0 < ngroups; g0 += NW * NIT) {
     T v[NIT][8];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      const int g = g0 + 4 * it < ngroups ? g0 + 4 * it : 0;
+      const int g = g0 + NW * it < ngroups ? g0 + NW * it : 0;
 #pragma unroll
       for (int i = 0; i < 8; ++i) v[it][i] = load(g, k8 * 8 + i, c);
     }
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
-      if (g0 + 4 * it >= ngroups) break;
-      const int g = slot(g0 + 4 * it);
+      if (g0 + NW * it >= ngroups) break;
+      const int g = slot(g0 + NW * it);
       int e = 0;
       if constexpr (Form<MODE>::kScaled) {
         T mx = 0;
@@ -251,11 +251,11 @@ __device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups
   }
 }
 
-// Groups a wave stages per memory round trip: its share of NGROUPS, at most
-// two, so the staged values (16 registers per group) stay few.
-template <int NGROUPS>
+// Groups a wave of NW stages per memory round trip: its share of NGROUPS, at
+// most two, so the staged values (16 registers per group) stay few.
+template <int NGROUPS, int NW>
 constexpr int stage_iters() {
-  return (NGROUPS + 3) / 4 < 2 ? (NGROUPS + 3) / 4 : 2;
+  return (NGROUPS + NW - 1) / NW < 2 ? (NGROUPS + NW - 1) / NW : 2;
 }
 
 template <int MODE>
@@ -276,9 +276,9 @@ constexpr int kOutPitch = 36;
 // LDS are sized by it, so small truncations keep a high occupancy.
 
 // Inverse: spec (FP64) -> four (FP32 even/odd Legendre sums).
-// Workgroup = one (m, CT 16-column tiles).  Each wave first issues the table
-// fragments of its first 16-pair row tile, then all 256 threads split the
-// spectral tiles into LDS; wave w then takes row tiles w, w+4, ... and runs
+// Workgroup = one (m, CT 16-column tiles) of NW waves.  Each wave first issues
+// the table fragments of its first 16-pair row tile, then all threads split
+// the spectral tiles into LDS; wave w then takes row tiles w, w+NW, ... and runs
 // each against every column tile with the same table fragments.  CT = 2
 // halves the workgroups, and with them the waves that wait on memory at
 // once, and the table fragments fetched per column.  Results leave through a
@@ -289,8 +289,8 @@ constexpr int kOutPitch = 36;
 // workgroups) is resident at once (measured at T85: neutral at 97 fields, 6 %
 // faster at 388; the six-product form spills two registers for it).  With
 // more slices the hint would spill heavily.
-template <int MODE, int NSL, int CT, bool NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((CT == 1 && NSL <= 2) ? 5 : 1))) void
+template <int MODE, int NSL, int CT, bool NT, int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu((CT == 1 && NSL <= 2) ? 5 : 1))) void
 leg_inv_mixed_kernel(
     const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
@@ -298,7 +298,7 @@ leg_inv_mixed_kernel(
   constexpr int NP = Form<MODE>::NP, NG = 2 * NSL;  // LDS groups per column tile
   __shared__ __align__(16) uint16_t bt[CT * NG * NP * 16 * kRowPitch];
   __shared__ int ebits[CT * NG * 16];
-  __shared__ __align__(16) float obuf[4 * 16 * kOutPitch];  // per-wave output transpose
+  __shared__ __align__(16) float obuf[NW * 16 * kOutPitch];  // per-wave output transpose
   int m, t;
   if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
   const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
@@ -332,7 +332,7 @@ leg_inv_mixed_kernel(
     const int j = g / ng, r = g - j * ng, par = r >= nsl;
     return j * NG + par * NSL + r - par * nsl;
   };
-  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * ng, slot, [&](int g, int k, int c) {
+  stage_data<MODE, stage_iters<CT * NG, NW>(), NW>(bt, ebits, nc * ng, slot, [&](int g, int k, int c) {
     const int j = g / ng, r = g - j * ng, par = r >= nsl, s = r - par * nsl;
     const int kk = s * 32 + k, cc = j * 16 + c;
     const bool v = kk < (par ? ko : ke) && ct0 * 16 + cc < ncol;
@@ -340,7 +340,7 @@ leg_inv_mixed_kernel(
     return v ? (float)x : 0.0f;  // split as FP32: see split(float)
   });
   __syncthreads();
-  for (int rt = wave; rt < rtiles; rt += 4) {
+  for (int rt = wave; rt < rtiles; rt += NW) {
     if (rt != wave) load_a(rt);
     const double ea0 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m]) : 1.0;
     const double ea1 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + 1]) : 1.0;
@@ -392,11 +392,12 @@ leg_inv_mixed_kernel(
 
 // Forward: four (FP32 S/A halves) -> spec (FP64).  Workgroup = one (m, CT
 // 16-column tiles); the S/A tiles (all latitude pairs, both parities) are
-// split once into LDS; wave w takes the (parity, 16-degree row tile) units w,
-// w+4, ... and runs each against every column tile with the same table
-// fragments.  NSL = P2p32 / 32 exactly (the dispatch picks it from the plan).
-template <int MODE, int NSL, int CT>
-__global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
+// split once into LDS; wave w of NW takes the (parity, 16-degree row tile)
+// units w, w+NW, ... and runs each against every column tile with the same
+// table fragments.  NSL = P2p32 / 32 exactly (the dispatch picks it from the
+// plan).
+template <int MODE, int NSL, int CT, int NW>
+__global__ __launch_bounds__(NW * 64) void leg_fwd_mixed_kernel(
     const float *__restrict__ four, double *__restrict__ spec, const uint16_t *__restrict__ L,
     const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
     int mlen, int N, int P2, int P2p32, int ncol, int nct, FourGeom geo) {
@@ -425,14 +426,14 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_kernel(
 #pragma unroll
       for (int q = 0; q < 3; ++q) pa[s][q] = (q < NP) ? a_frag(wave, s, q) : make_uint4(0, 0, 0, 0);
   }
-  stage_data<MODE, stage_iters<CT * NG>()>(bt, ebits, nc * NG, [](int g) { return g; }, [&](int g, int k, int c) {
+  stage_data<MODE, stage_iters<CT * NG, NW>(), NW>(bt, ebits, nc * NG, [](int g) { return g; }, [&](int g, int k, int c) {
     const int j = g / NG, par = (g % NG) / NSL, s = g % NSL, p = s * 32 + k, cc = ct0 * 16 + j * 16 + c;
     const bool v = p < P2 && cc < ncol;
     const float x = four[geo.at(v ? p : 0, m, par, v ? cc : ct0 * 16)];
     return v ? x : 0.0f;
   });
   __syncthreads();
-  for (int u = wave; u < nunits; u += 4) {
+  for (int u = wave; u < nunits; u += NW) {
     const int par = u / nrt, rt = u % nrt;
     const int kc = par ? (nn >> 1) : ke;
     if (u != wave) {
@@ -736,10 +737,12 @@ int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, h
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }
-  // KSPLIT: one column tile per workgroup; TILE: two
+  // KSPLIT: one column tile per workgroup; TILE: two.  Four waves (eight,
+  // as the forward takes them at T170, measured slower here: 242 against
+  // 217 us for TILE at T170, 776 fields).
   auto kern = shape == SHT_SHAPE_TILE
-                  ? (geo.stream ? leg_inv_mixed_kernel<MODE, NSL, 2, true> : leg_inv_mixed_kernel<MODE, NSL, 2, false>)
-                  : (geo.stream ? leg_inv_mixed_kernel<MODE, NSL, 1, true> : leg_inv_mixed_kernel<MODE, NSL, 1, false>);
+                  ? (geo.stream ? leg_inv_mixed_kernel<MODE, NSL, 2, true, 4> : leg_inv_mixed_kernel<MODE, NSL, 2, false, 4>)
+                  : (geo.stream ? leg_inv_mixed_kernel<MODE, NSL, 1, true, 4> : leg_inv_mixed_kernel<MODE, NSL, 1, false, 4>);
   const int ct = shape == SHT_SHAPE_TILE ? 2 : 1;
   launch_kernel(kern, xcd_m_blocks(p, 0, (nct + ct - 1) / ct), 256, 0, s, spec, four,
                 static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
@@ -770,9 +773,15 @@ int fwd_nsl(const Plan &p, int shape, const float *four, double *spec, int nf, h
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }
-  auto kern = shape == SHT_SHAPE_TILE ? leg_fwd_mixed_kernel<MODE, NSL, 2> : leg_fwd_mixed_kernel<MODE, NSL, 1>;
+  // Eight waves share the data tile from 128 latitude pairs (T170: TILE 171
+  // against 195 us at 776 fields, 26 against 31 us at 97; KSPLIT neutral),
+  // four below.
+  const bool eight = p.P2p >= 128;
+  auto kern = shape == SHT_SHAPE_TILE
+                  ? (eight ? leg_fwd_mixed_kernel<MODE, NSL, 2, 8> : leg_fwd_mixed_kernel<MODE, NSL, 2, 4>)
+                  : (eight ? leg_fwd_mixed_kernel<MODE, NSL, 1, 8> : leg_fwd_mixed_kernel<MODE, NSL, 1, 4>);
   const int ct = shape == SHT_SHAPE_TILE ? 2 : 1;
-  launch_kernel(kern, xcd_m_blocks(p, 1, (nct + ct - 1) / ct), 256, 0, s, four, spec,
+  launch_kernel(kern, xcd_m_blocks(p, 1, (nct + ct - 1) / ct), eight ? 512 : 256, 0, s, four, spec,
                 static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
                 p.xcd_mlen[1], p.N, p.P2, roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));
   SHT_HIP(hipGetLastError());
