@@ -893,7 +893,10 @@ int resolve_shape(const Plan &p, int nf, int dir) {
     if (e.nf == nf) return dir ? e.fwd : e.inv;
   const int d = dir ? p.shape_fwd : p.shape_inv;
   if (d != SHT_SHAPE_AUTO) return d;
-  if (p.prec != SHT_PREC_FP64) return SHT_SHAPE_KSPLIT;
+  // mixed forms: the shapes give bitwise equal results, so the choice is by
+  // speed alone -- two column tiles per workgroup from 128 latitude pairs
+  // (T170: faster in both directions at 97 to 776 fields), one below
+  if (p.prec != SHT_PREC_FP64) return p.P2p >= 128 ? SHT_SHAPE_TILE : SHT_SHAPE_KSPLIT;
   // measured on MI355X from T21 to T170 and 16 to 388 fields: the staged
   // shape wins the forward nearly everywhere and the inverse from 32 fields;
   // below that a wavefront per tile fills the chip better.  STAGED and WAVE
