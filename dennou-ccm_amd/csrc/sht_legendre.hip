@@ -859,6 +859,169 @@ int inv_staged_launch(const Plan &p, const double *spec, double *four, double *f
   return SHT_OK;
 }
 
+// Staged in two parts: the same shape for large batches on the T170 grid.
+// The K range (latitude pairs for the forward, degree steps of each parity
+// for the inverse) is staged and run in two halves, so the LDS block of half
+// the K range holds more column tiles -- two for the forward, four for the
+// inverse -- and each table fragment feeds that many MFMAs.  The steps of an
+// output tile still run in order from a zero accumulator, so results are
+// bitwise those of the one-part kernels; a wave's MAXU units keep their
+// accumulators across the halves and store after the second.  Measured at
+// T170 (eight waves): 776 fields forward 261 against 280 us, inverse 234
+// against 254 us; 388 fields 144 / 136.5 against 148 / 143.5 us; at 97 fields
+// slower (37-40 against 34-39 us), hence the batch threshold at dispatch.
+// Forward: LDS [CT][parity][P2p / 2 pairs][16].
+template <int CT, int D, int NW, int MAXU>
+__global__ __launch_bounds__(NW * 64) void leg_fwd_f64_halves_kernel(
+    const double *__restrict__ four, double *__restrict__ spec, const double *__restrict__ Pfwd,
+    const int *__restrict__ poff, const int *__restrict__ mlist, int mlen, int N, int P2, int P2p,
+    int ncol, int nct, FourGeom geo) {
+  extern __shared__ double sb[];
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
+  const int ct0 = t * CT, H = P2p >> 1;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KR = (ke + 15) & ~15, nrt = KR >> 4;
+  d4 acc[MAXU][CT];
+#pragma unroll
+  for (int i = 0; i < MAXU; ++i)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[i][c] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int h = 0; h < 2; ++h) {
+    if (h) __syncthreads();
+    stage_rows<4, NW * 64>(sb, 2 * H, CT, [&](int j, int r, int c) {
+      const int par = r / H, p = h * H + (r - par * H), col = (ct0 + j) * 16 + c;
+      return (p < P2 && col < ncol) ? *reinterpret_cast<const double2 *>(four + geo.at(p, m, par, col))
+                                    : make_double2(0.0, 0.0);
+    });
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXU; ++i) {
+      const int u = wave + NW * i, q = u & 1, rt = u >> 1;
+      if (u >= 2 * nrt) break;
+      const double *__restrict__ W = Pfwd + poff[m] + ((size_t)q * P2p + h * H) * KR + rt * 16 + r16;
+      const double *B = sb + (q * H + kq) * 16 + r16;
+      staged_unit<CT, D>(
+          H >> 2, [&](int s) { return W[(size_t)(s * 4 + kq) * KR]; },
+          [&](int c, int s) { return B[(c * 2 * H + s * 4) * 16]; }, acc[i]);
+    }
+  }
+  double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol;
+#pragma unroll
+  for (int i = 0; i < MAXU; ++i) {
+    const int u = wave + NW * i, q = u & 1, rt = u >> 1;
+    if (u >= 2 * nrt) break;
+    const int kn = q ? ko : ke;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int col = (ct0 + c) * 16 + r16;
+      if (col < ncol)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int k = rt * 16 + j * 4 + kq;
+          if (k < kn) __builtin_nontemporal_store(acc[i][c][j], sm + (size_t)(2 * k + q) * ncol + col);
+        }
+    }
+  }
+}
+
+// Inverse: steps S = KP / 4 of each parity split S0 = ceil(S / 2) + the rest;
+// LDS [CT][parity][4 S0 rows][16].
+template <int CT, int D, int NW, int MAXU, bool NT>
+__global__ __launch_bounds__(NW * 64) void leg_inv_f64_halves_kernel(
+    const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
+    const double *__restrict__ Pinv, const double *__restrict__ Hinv, const int *__restrict__ poff,
+    const int *__restrict__ mlist, int mlen, int N, int P2, int P2p, int ncol, int nct, int tab0,
+    int ntab, FourGeom geo) {
+  extern __shared__ double sb[];
+  const int ncg = (nct + CT - 1) / CT;
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, ntab * ncg, m, t)) return;
+  const int tab = tab0 + t % ntab, ct0 = (t / ntab) * CT;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP = (ke + 3) & ~3, S = KP >> 2, S0 = (S + 1) >> 1;
+  const double *__restrict__ sp = spec + (size_t)coef_off(N, m) * ncol;
+  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int rtiles = P2p >> 4;
+  double *__restrict__ out = tab ? four2 : four;
+  const int se = tab ? 1 : 0;  // the derivative table's parities are swapped
+  d4 acc[MAXU][CT];
+#pragma unroll
+  for (int i = 0; i < MAXU; ++i)
+#pragma unroll
+    for (int c = 0; c < CT; ++c) acc[i][c] = d4{0.0, 0.0, 0.0, 0.0};
+  for (int h = 0; h < 2; ++h) {
+    const int slo = h ? S0 : 0, ns = h ? S - S0 : S0, rows = 4 * ns;
+    if (ns == 0) break;
+    if (h) __syncthreads();
+    stage_rows<4, NW * 64>(sb, 2 * rows, CT, [&](int j, int r, int c) {
+      const int par = r / rows, k = 4 * slo + (r - par * rows), col = (ct0 + j) * 16 + c;
+      if (k >= (par ? ko : ke) || col >= ncol) return make_double2(0.0, 0.0);
+      const double *x = sp + (size_t)(2 * k + par) * ncol + col;
+      return make_double2(x[0], x[1]);
+    });
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < MAXU; ++i) {
+      const int u = wave + NW * i, q = u & 1, rt = u >> 1;
+      if (u >= 2 * rtiles) break;
+      const double *__restrict__ T =
+          (tab ? Hinv : Pinv) + poff[m] + ((size_t)q * KP + 4 * slo) * P2p + rt * 16 + r16;
+      const double *B = sb + (q * rows + kq) * 16 + r16;
+      staged_unit<CT, D>(
+          ns, [&](int s) { return T[(size_t)(s * 4 + kq) * P2p]; },
+          [&](int c, int s) { return B[(c * 2 * rows + s * 4) * 16]; }, acc[i]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < MAXU; ++i) {
+    const int u = wave + NW * i, q = u & 1, rt = u >> 1;
+    if (u >= 2 * rtiles) break;
+    const int slot = q ^ se;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+      const int col = (ct0 + c) * 16 + r16;
+      if (col < ncol)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int p = rt * 16 + j * 4 + kq;
+          if (p < P2) put_four<NT>(out + geo.at(p, m, slot, col), acc[i][c][j]);
+        }
+    }
+  }
+}
+
+// Field count from which the staged shape runs in two parts on the T170 grid.
+constexpr int kHalvesMinFields = 192;
+
+// MAXU = 2: at most 2 nrt(0) <= 16 forward units, and 2 (P2p / 16) = 16
+// inverse units at P2p = 128, over eight waves.
+template <int CT, int MAXU>
+int fwd_halves_launch(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const size_t lds = sizeof(double) * CT * 2 * (p.P2p / 2) * 16;
+  launch_kernel(leg_fwd_f64_halves_kernel<CT, 4, 8, MAXU>, xcd_m_blocks(p, 1, (nct + CT - 1) / CT), 512, lds,
+      s, four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
+      four_geom(p.N, p.im, nf));
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+template <int CT, int MAXU>
+int inv_halves_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0,
+                      int ntab, int nf, hipStream_t s) {
+  const int ncol = 2 * nf, nct = (ncol + 15) / 16;
+  const int S = roundup(ceil_half(p.N + 1), 4) / 4, S0 = (S + 1) / 2;
+  const size_t lds = sizeof(double) * CT * 2 * (4 * S0) * 16;
+  const FourGeom geo = four_geom_out(p, nf, sizeof(double));
+  launch_kernel(geo.stream ? leg_inv_f64_halves_kernel<CT, 4, 8, MAXU, true>
+                           : leg_inv_f64_halves_kernel<CT, 4, 8, MAXU, false>,
+      xcd_m_blocks(p, 0, ntab * ((nct + CT - 1) / CT)), 512, lds, s, spec, four, four2, p.Pinv, p.Hinv,
+      p.poff_inv, p.xcd_m[0], p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, tab0, ntab, geo);
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 template <int RT>
 int inv_launch(const Plan &p, const double *spec, double *four, double *four2, int tab0, int ntab,
                int nf, hipStream_t s) {
@@ -913,6 +1076,9 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
   const int tab0 = value_table ? 0 : 1;
   const int ntab = (value_table && deriv_table) ? 2 : 1;
   if (shape == SHT_SHAPE_WAVE) return inv_wave_launch<4>(p, spec, four, four2, tab0, ntab, nf, s);
+  // two-part staging for large batches on the 128-pair grid (T170)
+  if (shape == SHT_SHAPE_STAGED && p.P2p == 128 && nf >= kHalvesMinFields)
+    return inv_halves_launch<4, 2>(p, spec, four, four2, tab0, ntab, nf, s);
   if (shape == SHT_SHAPE_STAGED)
     return p.P2p >= 128 ? inv_staged_launch<2, 8>(p, spec, four, four2, tab0, ntab, nf, s)
                         : inv_staged_launch<1, 4>(p, spec, four, four2, tab0, ntab, nf, s);
@@ -951,6 +1117,8 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
 int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
                              hipStream_t s) {
   if (shape == SHT_SHAPE_WAVE) return fwd_wave_launch<4>(p, four, spec, nf, s);
+  if (shape == SHT_SHAPE_STAGED && p.P2p == 128 && nf >= kHalvesMinFields)
+    return fwd_halves_launch<2, 2>(p, four, spec, nf, s);
   if (shape == SHT_SHAPE_STAGED)
     return p.P2p >= 128 ? fwd_staged_launch<1, 8>(p, four, spec, nf, s) : fwd_staged_launch<1, 4>(p, four, spec, nf, s);
   const int rt = (ceil_half(p.N + 1) + 15) / 16;

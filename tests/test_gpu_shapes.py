@@ -124,7 +124,8 @@ def test_shape_results_do_not_depend_on_batch():
     p.close()
 
 
-@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (42, 128, 64, 9), (170, 512, 256, 13)])
+@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (42, 128, 64, 9), (170, 512, 256, 13),
+                                         (170, 512, 256, 200)])  # 200: the staged shape in two K parts
 def test_wave_tile_staged_bitwise_equal(N, im, jm, nf):
     """WAVE, TILE and STAGED run the same MFMA sequence per output tile of the
     inverse (steps in order, padded steps adding exact zeros), and WAVE and
