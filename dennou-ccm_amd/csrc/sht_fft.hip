@@ -202,18 +202,23 @@ struct FwdIO {
 // threads first load whole rows into LDS (runs of 64 samples per wave), which
 // streams better when blocks queue.  The last pass writes only the
 // wavenumbers m < M1 and L - m the Legendre step reads.
+// spread (both FFT kernels): the launch fits the device at once, so blocks
+// take tiles in hardware order (block b runs on XCD b mod 8), which deals the
+// workgroups of the last, partly empty field group evenly over the XCDs (T85,
+// 97 fields: inverse 9.8 -> 9.5 us); otherwise each XCD takes a contiguous
+// range of tiles, which measured ~1 % faster when blocks queue.
 template <int L, typename TF, bool DIRECT>
 __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
                                                            const double2 *__restrict__ tw,
                                                            const double *__restrict__ coslat,
-                                                           int P2, int M1, int nf, int jm) {
+                                                           int P2, int M1, int nf, int jm, int spread) {
   const double *__restrict__ grid = io.src[blockIdx.y];
   TF *__restrict__ four = io.dst[blockIdx.y];
   const int mode = io.mode[blockIdx.y];
   constexpr int G = kLdsComplex / L;
   __shared__ double2 buf[kLdsPadded];
   __shared__ double2 twl[L];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = spread ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
   const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
@@ -286,12 +291,12 @@ template <int L, typename TF>
 __global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
                                                            const double2 *__restrict__ tw,
                                                            const double *__restrict__ coslat,
-                                                           int P2, int M1, int nf, int jm) {
+                                                           int P2, int M1, int nf, int jm, int spread) {
   constexpr int G = kLdsComplex / L;
   constexpr int NM = (kLdsComplex / 2) / kThreads; // max (m, field) pairs per thread
   __shared__ double2 buf[kLdsPadded];
   __shared__ double2 twl[L];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = spread ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
   const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
@@ -354,9 +359,10 @@ template <int L, typename TF>
 int fwd_dispatch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  auto kern = nblk * nin <= p.fft_fwd_resident ? fft_fwd_kernel<L, TF, true> : fft_fwd_kernel<L, TF, false>;
+  const bool resident = nblk * nin <= p.fft_fwd_resident;
+  auto kern = resident ? fft_fwd_kernel<L, TF, true> : fft_fwd_kernel<L, TF, false>;
   launch_kernel(kern, dim3(nblk, nin), kThreads, 0, s,
-      io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm);
+      io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm, resident ? 1 : 0);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -375,8 +381,10 @@ template <int L, typename TF>
 int inv_dispatch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
   constexpr int G = kLdsComplex / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
+  // same LDS and block size as the forward kernel, hence the same residency
+  const bool resident = nblk * nout <= p.fft_fwd_resident;
   launch_kernel(fft_inv_kernel<L, TF>, dim3(nblk, nout), kThreads, 0, s,
-      io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm);
+      io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm, resident ? 1 : 0);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
