@@ -159,6 +159,27 @@ def run_gpu(precision: int, steps: int, warmup: int, autotune: bool = True):
             "grid": out_grid, "spec": out_spec, "spec_in": spec_c, "shapes": shapes}
 
 
+def host_cpu_info():
+    """CPU model, sockets and logical CPUs of this host (from /proc/cpuinfo),
+    and the CPUs this process may run on."""
+    model, sockets, logical = None, set(), 0
+    try:
+        for line in open("/proc/cpuinfo"):
+            key, _, val = line.partition(":")
+            key, val = key.strip(), val.strip()
+            if key == "model name" and model is None:
+                model = val
+            elif key == "physical id":
+                sockets.add(val)
+            elif key == "processor":
+                logical += 1
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": model, "sockets": len(sockets) or None, "logical_cpus": logical or None,
+            "usable_cpus": usable}
+
+
 def cpu_baseline(spec_c, seconds: float, threads: int):
     import oracle
 
@@ -215,6 +236,20 @@ def run_serve(precision: str, rates, nreq_per_rate, slots: int = 4, max_fields: 
         del data
         torch.cuda.empty_cache()
     return out
+
+
+def copy_calibration(nbytes: int = 64 << 20, reps: int = 3):
+    """A plain device copy of known size: under rocprofv3 --pmc FETCH_SIZE and
+    --pmc WRITE_SIZE, what the counters read per dispatch against the bytes
+    actually moved gives the factors that turn them into HBM bytes."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"workload": "copycal", "bytes_per_copy": nbytes, "copies": reps}
 
 
 def traffic_from_profile(form: str, kernel_prefix: str):
@@ -320,9 +355,10 @@ def main():
     ap.add_argument("--precision", default="auto", choices=sorted(PRECISIONS) + ["auto"],
                     help="auto: run every form; the headline is split-BF16 (six products, <=1e-6)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve"],
+    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve", "copycal"],
                     help="ensemble: the 64-member mixed-truncation batch; serve: request streams at "
-                         "100-1000 req/s (separate JSON lines)")
+                         "100-1000 req/s (separate JSON lines); copycal: a plain 64 MiB device copy, "
+                         "three dispatches, the calibration run for rocprofv3's FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests in the 500 req/s serving run of the headline line (0 = skip)")
     ap.add_argument("--members", type=int, default=64)
@@ -334,6 +370,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
                                    "(the transform would shard by field batch, one slab of levels per GPU)"}))
         sys.exit(2)
+
+    if args.workload == "copycal":
+        print(json.dumps(copy_calibration()))
+        return
 
     import oracle
 
@@ -407,7 +447,8 @@ def main():
     if args.serve_requests > 0:
         serve_res = run_serve(headline, [500], lambda r: args.serve_requests)["rates"]["500"]
 
-    threads = min(16, os.cpu_count() or 1)
+    cpu = host_cpu_info()
+    threads = cpu["usable_cpus"]  # every core this process may run on
     cpu_v, cpu_n, cpu_dt = cpu_baseline(res["spec_in"], args.cpu_seconds, threads)
     cpu1_v, cpu1_n, cpu1_dt = cpu_baseline(res["spec_in"], max(2.0, args.cpu_seconds / 4), 1)
 
@@ -436,8 +477,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": {"value": round(cpu_v, 1), "unit": "fields/s", "cores": threads, "kind": "port",
                          "sample": f"oracle round trips of the same 97-field T85L32 batch, {cpu_n} reps "
-                                   f"in {cpu_dt:.1f} s, {threads} OpenMP threads",
-                         "single_core_value": round(cpu1_v, 1)},
+                                   f"in {cpu_dt:.1f} s, {threads} OpenMP threads (one per usable CPU)",
+                         "single_core_value": round(cpu1_v, 1), "host": cpu},
     }
     print(json.dumps(line))
 

@@ -26,7 +26,7 @@ FLAGS = [
 ]
 SOURCES = ["sht_capi.hip", "sht_tables.hip", "sht_fft.hip", "sht_legendre.hip",
            "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip", "sht_server.hip",
-           "sht_tune.hip", "sht_remap.hip"]
+           "sht_tune.hip", "sht_remap.hip", "sht_zonal.hip", "sht_exchange.hip"]
 
 
 def _compile(src: str) -> str:
@@ -40,30 +40,33 @@ def _compile(src: str) -> str:
 
 FORTRAN = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
 FDIR = os.path.join(HERE, "fortran")
-FTEST = os.path.join(ROOT, "tests", "fortran", "test_wa_module.f90")
+FTESTS = os.path.join(ROOT, "tests", "fortran")
+FMODULES = ("wa_module", "l_module")
 
 
 def build_fortran(verbose: bool = False) -> str | None:
-    """Compiles the Fortran wa_module (ISO_C_BINDING over the C ABI) into
-    lib/wa_module.o + lib/wa_module.mod and links the drop-in test program
-    lib/test_wa_module against libsht_hip.so.  Skipped without amdflang."""
+    """Compiles the Fortran drop-in modules (wa_module, l_module; ISO_C_BINDING
+    over the C ABI) into lib/<module>.o + .mod and links each drop-in test
+    program lib/test_<module> against libsht_hip.so.  Skipped without amdflang."""
     if not os.path.exists(FORTRAN):
         return None
     libdir = os.path.dirname(LIB)
-    obj = os.path.join(libdir, "wa_module.o")
-    exe = os.path.join(libdir, "test_wa_module")
-    steps = [
-        [FORTRAN, "-O2", "-fPIC", "-c", os.path.join(FDIR, "wa_module.f90"), "-module-dir", libdir, "-o", obj],
-        [FORTRAN, "-O2", FTEST, obj, "-module-dir", libdir, "-I", libdir, "-L", libdir, "-lsht_hip",
-         "-Wl,-rpath,$ORIGIN", "-o", exe],
-    ]
+    steps = []
+    for mod in FMODULES:
+        obj = os.path.join(libdir, f"{mod}.o")
+        exe = os.path.join(libdir, f"test_{mod}")
+        steps += [
+            [FORTRAN, "-O2", "-fPIC", "-c", os.path.join(FDIR, f"{mod}.f90"), "-module-dir", libdir, "-o", obj],
+            [FORTRAN, "-O2", os.path.join(FTESTS, f"test_{mod}.f90"), obj, "-module-dir", libdir, "-I", libdir,
+             "-L", libdir, "-lsht_hip", "-Wl,-rpath,$ORIGIN", "-o", exe],
+        ]
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True, cwd=libdir)
         if r.returncode != 0:
             raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stderr}")
     if verbose:
-        print("built", exe)
-    return exe
+        print("built the Fortran drop-in modules and their test programs in", libdir)
+    return libdir
 
 
 def build(verbose: bool = False) -> str:

@@ -215,6 +215,9 @@ int sht_batch_run(sht_batch *b, int dir, void *stream) {
 
 int sht_batch_autotune(sht_batch *b, int reps, const char *cache_path) {
   if (!b) return fail(SHT_ERR_INVALID, "sht_batch_autotune: batch is NULL");
+  // the tuner runs on a stream of its own through each group plan's Fourier
+  // scratch: a graph or eager run still in flight on any stream would share it
+  SHT_HIP(hipDeviceSynchronize());
   for (auto &g : b->groups) {
     int rc = sht_plan_autotune(g.plan, g.nf, reps, cache_path, nullptr);
     if (rc) return rc;
@@ -226,6 +229,7 @@ int sht_batch_autotune(sht_batch *b, int reps, const char *cache_path) {
     b->exec[d] = nullptr;
     b->graph[d] = nullptr;
   }
+  SHT_HIP(hipDeviceSynchronize());
   return SHT_OK;
 }
 

@@ -172,6 +172,8 @@ inline void launch_kernel(void (*kernel)(KArgs...), dim3 grid, dim3 block, size_
 // ------------------------------------------------------------------ launchers
 // Table construction (sht_tables.hip)
 int build_tables(Plan &p);
+// Gauss-Legendre nodes mu (S->N), weights and cos(lat) of jm (even) latitudes
+int launch_gauss(int jm, double *mu, double *w, double *coslat, hipStream_t s);
 
 // FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos).
 // f32 = the Fourier intermediate is FP32 (mixed-precision plans); the FFT itself is FP64.

@@ -1052,8 +1052,13 @@ int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStrea
 }  // namespace
 
 int resolve_shape(const Plan &p, int nf, int dir) {
+  // an entry for nf, unless its half for this direction is AUTO
   for (const Plan::ShapeEntry &e : p.shapes)
-    if (e.nf == nf) return dir ? e.fwd : e.inv;
+    if (e.nf == nf) {
+      const int s = dir ? e.fwd : e.inv;
+      if (s != SHT_SHAPE_AUTO) return s;
+      break;
+    }
   const int d = dir ? p.shape_fwd : p.shape_inv;
   if (d != SHT_SHAPE_AUTO) return d;
   // mixed forms: the shapes give bitwise equal results, so the choice is by

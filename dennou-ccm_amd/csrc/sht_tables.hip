@@ -143,6 +143,12 @@ __global__ void twiddle_kernel(int L, double *tw) {
 
 }  // namespace
 
+int launch_gauss(int jm, double *mu, double *w, double *coslat, hipStream_t s) {
+  gauss_kernel<<<(jm / 2 + 63) / 64, 64, 0, s>>>(jm, mu, w, coslat);
+  SHT_HIP(hipGetLastError());
+  return SHT_OK;
+}
+
 int build_tables(Plan &p) {
   const int N = p.N, P2 = p.P2, P2p = p.P2p;
   p.h_poff_inv.resize(N + 1);
@@ -210,8 +216,8 @@ int build_tables(Plan &p) {
     SHT_HIP(hipMemcpy(p.xcd_m[dir], flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
   }
 
-  gauss_kernel<<<(P2 + 63) / 64, 64>>>(p.jm, p.mu, p.w, p.coslat);
-  SHT_HIP(hipGetLastError());
+  int rc = launch_gauss(p.jm, p.mu, p.w, p.coslat, nullptr);
+  if (rc) return rc;
   int nthr = P2 * (N + 1);
   legendre_kernel<<<(nthr + 127) / 128, 128>>>(N, P2, P2p, p.mu, p.coslat, p.w, p.poff_inv,
                                                  p.poff_fwd, p.Pinv, p.Pfwd, p.Hinv, p.Hfwd);

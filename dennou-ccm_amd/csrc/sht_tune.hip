@@ -119,9 +119,9 @@ int sht_plan_set_shapes(sht_plan *h, int nf, int inv, int fwd) {
                                   [nf](const Plan::ShapeEntry &e) { return e.nf == nf; }),
                    p.shapes.end());
   } else {
-    // an AUTO half of an entry resolves to the plan default at call time
-    put_entry(p, nf, inv == SHT_SHAPE_AUTO ? resolve_shape(p, 0, 0) : inv,
-              fwd == SHT_SHAPE_AUTO ? resolve_shape(p, 0, 1) : fwd);
+    // an AUTO half stays AUTO: at call time it resolves to the plan default,
+    // else to the built-in choice for nf fields (resolve_shape)
+    put_entry(p, nf, inv, fwd);
   }
   return SHT_OK;
 }

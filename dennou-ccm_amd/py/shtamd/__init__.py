@@ -111,6 +111,15 @@ SIGNATURES = {
     "sht_plan_set_shapes": (_ci, [_vp, _ci, _ci, _ci]),
     "sht_plan_get_shapes": (_ci, [_vp, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_plan_autotune": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _dp]),
+    "sht_zonal_create": (_ci, [_ci, _ci, _ci, ctypes.POINTER(_vp)]),
+    "sht_zonal_destroy": (None, [_vp]),
+    "sht_zonal_grid": (_ci, [_vp, _dp, _dp]),
+    "sht_l_y_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_y_l_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_l_y": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_y_l": (_ci, [_vp, _dp, _dp, _ci]),
+    "sht_lat_cell_edges": (_ci, [_dp, _ci, _dp]),
+    "sht_exchange_grid": (_ci, [_dp, _dp, _ci, _dp, _dp, _ci, _dp, _dp, ctypes.POINTER(_ci)]),
 }
 
 STAGES = ("leg_inv", "fft_inv", "fft_fwd", "leg_fwd")
@@ -147,6 +156,20 @@ def _ptr(t) -> int | None:
     if not t.is_contiguous():
         raise ValueError("tensor must be contiguous")
     return t.data_ptr()
+
+
+def _dev(t, need: int, what: str) -> int | None:
+    """Device address of a float64 CUDA tensor holding at least `need`
+    elements; None passes through (optional outputs)."""
+    if t is None:
+        return None
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{what}: expected a CUDA tensor")
+    if str(t.dtype) != "torch.float64":
+        raise ValueError(f"{what}: expected float64, got {t.dtype}")
+    if t.numel() < need:
+        raise ValueError(f"{what}: {t.numel()} elements, need at least {need}")
+    return _ptr(t)
 
 
 def _stream_handle(stream) -> int | None:
@@ -194,33 +217,49 @@ class Plan:
         return lon, lat, lw, yw
 
     # ---- device (canonical spectra, stream ordered)
+    def _ns(self, nfield):
+        return self.K * 2 * nfield
+
+    def _ng(self, nfield):
+        return nfield * self.jm * self.im
+
     def spec_to_grid(self, spec, grid, nfield: int, stream=None):
-        check(lib().sht_spec_to_grid_dev(self._h, _ptr(spec), _ptr(grid), nfield, _stream_handle(stream)))
+        check(lib().sht_spec_to_grid_dev(self._h, _dev(spec, self._ns(nfield), "spec"),
+                                         _dev(grid, self._ng(nfield), "grid"), nfield, _stream_handle(stream)))
 
     def grid_to_spec(self, grid, spec, nfield: int, stream=None):
-        check(lib().sht_grid_to_spec_dev(self._h, _ptr(grid), _ptr(spec), nfield, _stream_handle(stream)))
+        check(lib().sht_grid_to_spec_dev(self._h, _dev(grid, self._ng(nfield), "grid"),
+                                         _dev(spec, self._ns(nfield), "spec"), nfield, _stream_handle(stream)))
 
     def spec_to_grad(self, spec, grid, gradlon, gradlat, nfield: int, stream=None):
-        check(lib().sht_spec_to_grad_dev(self._h, _ptr(spec), _ptr(grid), _ptr(gradlon), _ptr(gradlat),
+        ng = self._ng(nfield)
+        check(lib().sht_spec_to_grad_dev(self._h, _dev(spec, self._ns(nfield), "spec"), _dev(grid, ng, "grid"),
+                                         _dev(gradlon, ng, "gradlon"), _dev(gradlat, ng, "gradlat"),
                                          nfield, _stream_handle(stream)))
 
     def grid_to_div(self, a, b, spec, nfield: int, stream=None):
         """Spectra of (1/cos) dA/dlon + (1/cos) d(B cos)/dlat; a or b may be None."""
-        check(lib().sht_grid_to_div_dev(self._h, _ptr(a), _ptr(b), _ptr(spec), nfield,
-                                        _stream_handle(stream)))
+        ng = self._ng(nfield)
+        check(lib().sht_grid_to_div_dev(self._h, _dev(a, ng, "a"), _dev(b, ng, "b"),
+                                        _dev(spec, self._ns(nfield), "spec"), nfield, _stream_handle(stream)))
 
     def spec_lapla(self, spec_in, spec_out, power: int, nfield: int, stream=None):
-        check(lib().sht_spec_lapla_dev(self._h, _ptr(spec_in), _ptr(spec_out), power, nfield,
-                                       _stream_handle(stream)))
+        ns = self._ns(nfield)
+        check(lib().sht_spec_lapla_dev(self._h, _dev(spec_in, ns, "spec_in"), _dev(spec_out, ns, "spec_out"),
+                                       power, nfield, _stream_handle(stream)))
 
     def spml_to_canon(self, wa, spec, nfield: int, stream=None):
-        check(lib().sht_spml_to_canon_dev(self._h, _ptr(wa), _ptr(spec), nfield, _stream_handle(stream)))
+        check(lib().sht_spml_to_canon_dev(self._h, _dev(wa, nfield * (self.nmax + 1) ** 2, "wa"),
+                                          _dev(spec, self._ns(nfield), "spec"), nfield, _stream_handle(stream)))
 
     def canon_to_spml(self, spec, wa, nfield: int, stream=None):
-        check(lib().sht_canon_to_spml_dev(self._h, _ptr(spec), _ptr(wa), nfield, _stream_handle(stream)))
+        check(lib().sht_canon_to_spml_dev(self._h, _dev(spec, self._ns(nfield), "spec"),
+                                          _dev(wa, nfield * (self.nmax + 1) ** 2, "wa"), nfield,
+                                          _stream_handle(stream)))
 
     def avr_lonlat(self, grid, avg, nfield: int, stream=None):
-        check(lib().sht_avr_lonlat_dev(self._h, _ptr(grid), _ptr(avg), nfield, _stream_handle(stream)))
+        check(lib().sht_avr_lonlat_dev(self._h, _dev(grid, self._ng(nfield), "grid"), _dev(avg, nfield, "avg"),
+                                       nfield, _stream_handle(stream)))
 
     # ---- stage timers
     def set_timing(self, enable: bool):
@@ -252,35 +291,118 @@ class Plan:
         return self.get_shapes(nfield), (us[0], us[1])
 
     # ---- host drop-in (SPML packing), numpy arrays
-    def _host_call(self, fn, inp, out_shape):
+    def _host_call(self, fn, inp, in_tail, out_tail):
         inp = np.ascontiguousarray(inp, dtype=np.float64)
-        out = np.empty(out_shape)
+        if inp.ndim != 1 + len(in_tail) or inp.shape[1:] != tuple(in_tail):
+            raise ValueError(f"expected an array of shape (nlev, {', '.join(map(str, in_tail))}), "
+                             f"got {inp.shape}")
+        out = np.empty((inp.shape[0], *out_tail))
         check(fn(self._h, _host(inp), _host(out), inp.shape[0]))
         return out
 
+    def _wa(self):
+        return ((self.nmax + 1) ** 2,)
+
+    def _xy(self):
+        return (self.jm, self.im)
+
     def xya_wa(self, wa):
-        return self._host_call(lib().sht_xya_wa, wa, (wa.shape[0], self.jm, self.im))
+        return self._host_call(lib().sht_xya_wa, wa, self._wa(), self._xy())
 
     def wa_xya(self, xya):
-        return self._host_call(lib().sht_wa_xya, xya, (xya.shape[0], (self.nmax + 1) ** 2))
+        return self._host_call(lib().sht_wa_xya, xya, self._xy(), self._wa())
 
     def xya_gradlon_wa(self, wa):
-        return self._host_call(lib().sht_xya_gradlon_wa, wa, (wa.shape[0], self.jm, self.im))
+        return self._host_call(lib().sht_xya_gradlon_wa, wa, self._wa(), self._xy())
 
     def xya_gradlat_wa(self, wa):
-        return self._host_call(lib().sht_xya_gradlat_wa, wa, (wa.shape[0], self.jm, self.im))
+        return self._host_call(lib().sht_xya_gradlat_wa, wa, self._wa(), self._xy())
 
     def wa_divlon_xya(self, xya):
-        return self._host_call(lib().sht_wa_divlon_xya, xya, (xya.shape[0], (self.nmax + 1) ** 2))
+        return self._host_call(lib().sht_wa_divlon_xya, xya, self._xy(), self._wa())
 
     def wa_divlat_xya(self, xya):
-        return self._host_call(lib().sht_wa_divlat_xya, xya, (xya.shape[0], (self.nmax + 1) ** 2))
+        return self._host_call(lib().sht_wa_divlat_xya, xya, self._xy(), self._wa())
 
     def wa_lapla_wa(self, wa):
-        return self._host_call(lib().sht_wa_lapla_wa, wa, wa.shape)
+        return self._host_call(lib().sht_wa_lapla_wa, wa, self._wa(), self._wa())
 
     def wa_laplainv_wa(self, wa):
-        return self._host_call(lib().sht_wa_laplainv_wa, wa, wa.shape)
+        return self._host_call(lib().sht_wa_laplainv_wa, wa, self._wa(), self._wa())
+
+
+class Zonal:
+    """Zonal (m = 0) Legendre transform (SPML l_module, l_Initial(nm, jm)).
+    Grids y (nfield, jm) S->N, Legendre coefficients l (nfield, nmax + 1);
+    l_y: l_n = (1/2) sum_j w_j Pt_n(mu_j) y_j, y_l its inverse, Pt_n = sqrt(2n+1) P_n."""
+
+    def __init__(self, nmax: int, jm: int, kmax: int = 1):
+        h = _vp()
+        check(lib().sht_zonal_create(nmax, jm, kmax, ctypes.byref(h)))
+        self._h = h
+        self.nmax, self.jm, self.kmax = nmax, jm, kmax
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().sht_zonal_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid(self):
+        """(y_Lat, y_Lat_Weight)"""
+        lat, w = np.empty(self.jm), np.empty(self.jm)
+        check(lib().sht_zonal_grid(self._h, _host(lat), _host(w)))
+        return lat, w
+
+    def l_y_dev(self, y, l, nfield: int, stream=None):
+        check(lib().sht_l_y_dev(self._h, _dev(y, nfield * self.jm, "y"), _dev(l, nfield * (self.nmax + 1), "l"),
+                                nfield, _stream_handle(stream)))
+
+    def y_l_dev(self, l, y, nfield: int, stream=None):
+        check(lib().sht_y_l_dev(self._h, _dev(l, nfield * (self.nmax + 1), "l"), _dev(y, nfield * self.jm, "y"),
+                                nfield, _stream_handle(stream)))
+
+    def _host(self, fn, inp, n_in, n_out):
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        one = inp.ndim == 1
+        if one:
+            inp = inp[None]
+        if inp.ndim != 2 or inp.shape[1] != n_in:
+            raise ValueError(f"expected shape ({n_in},) or (nlev, {n_in}), got {inp.shape}")
+        out = np.empty((inp.shape[0], n_out))
+        check(fn(self._h, _host(inp), _host(out), inp.shape[0]))
+        return out[0] if one else out
+
+    def l_y(self, y):
+        return self._host(lib().sht_l_y, y, self.jm, self.nmax + 1)
+
+    def y_l(self, l):
+        return self._host(lib().sht_y_l, l, self.nmax + 1, self.jm)
+
+
+def lat_cell_edges(lat_weight):
+    """Cell edges of a latitude set (gmapgen_main.f90:364-368): jm + 1 values."""
+    w = np.ascontiguousarray(lat_weight, dtype=np.float64)
+    e = np.empty(w.size + 1)
+    check(lib().sht_lat_cell_edges(_host(w), w.size, _host(e)))
+    return e
+
+
+def exchange_grid(lat_a, w_a, lat_b, w_b):
+    """gmapgen's surface exchange grid of latitude sets A and B: (lat, weight)."""
+    la, wa = (np.ascontiguousarray(x, dtype=np.float64) for x in (lat_a, w_a))
+    lb, wb = (np.ascontiguousarray(x, dtype=np.float64) for x in (lat_b, w_b))
+    cap = wa.size + wb.size - 1
+    lat, w, n = np.empty(cap), np.empty(cap), _ci()
+    check(lib().sht_exchange_grid(_host(la), _host(wa), wa.size, _host(lb), _host(wb), wb.size,
+                                  _host(lat), _host(w), ctypes.byref(n)))
+    return lat[:n.value].copy(), w[:n.value].copy()
 
 
 class Batch:

@@ -144,6 +144,40 @@ int sht_wa_divlat_xya(sht_plan *plan, const double *xya, double *wa, int nlev); 
 int sht_wa_lapla_wa(sht_plan *plan, const double *wa, double *wa_out, int nlev);     /* on the host */
 int sht_wa_laplainv_wa(sht_plan *plan, const double *wa, double *wa_out, int nlev);  /* on the host */
 
+/* ---- zonal transform: SPML l_module (l_Initial(nm, jm), y_Lat, y_Lat_Weight,
+ *      l_y, y_l) as the energy-balance model uses it
+ *      (exp/APEI07Couple/ebm/EBM_main.f90:64 l_initial(1024, 1024); l_y at
+ *      :170 and :242).  m = 0 only:
+ *        l_y:  l_n = (1/2) sum_j w_j Pt_n(mu_j) y_j,   y_l:  y_j = sum_n l_n Pt_n(mu_j),
+ *      Pt_n = sqrt(2n+1) P_n.  nmax may equal jm (degree jm vanishes at every
+ *      Gauss node).  Layouts: y[f][j] (jm per field, S->N), l[f][n] (nmax+1
+ *      per field).  FP64 on v_mfma_f64_16x16x4f64.                          */
+typedef struct sht_zonal sht_zonal;
+int sht_zonal_create(int nmax, int jm, int kmax, sht_zonal **zonal);
+void sht_zonal_destroy(sht_zonal *zonal);
+/* y_Lat (radians, S->N) and y_Lat_Weight (sum 2); host arrays of jm, either may be NULL */
+int sht_zonal_grid(const sht_zonal *zonal, double *lat, double *lat_weight);
+/* device, stream ordered */
+int sht_l_y_dev(sht_zonal *zonal, const double *y, double *l, int nfield, void *stream);
+int sht_y_l_dev(sht_zonal *zonal, const double *l, double *y, int nfield, void *stream);
+/* host, synchronous (nlev <= kmax) */
+int sht_l_y(sht_zonal *zonal, const double *y, double *l, int nlev);
+int sht_y_l(sht_zonal *zonal, const double *l, double *y, int nlev);
+
+/* ---- latitude cells and the surface exchange grid of the coupler's grid
+ *      generator (tool/gmapgen/gmapgen_main.f90:336-405; the same cells at
+ *      exp/APEI07Couple/ebm/EBM_main.f90:200-204).  Host arithmetic over the
+ *      Gauss weights of a plan (sht_plan_grid / sht_zonal_grid), S->N.      */
+/* edges[0] = -pi/2, edges[j] = asin(w[j-1] + sin(edges[j-1])), edges[jm] = pi/2 */
+int sht_lat_cell_edges(const double *lat_weight, int jm, double *edges);
+/* Merges the cell edges of grids A and B (A's edges 0..jm_a-1, B's 1..jm_b),
+ * sorts them and keeps each band of area > 1e-12 as one latitude (band
+ * midpoint) with its area as weight; jm_a == jm_b returns grid A.  lat and
+ * lat_weight hold up to jm_a + jm_b - 1 values; *jm_out receives the count.
+ * lat_b may be NULL. */
+int sht_exchange_grid(const double *lat_a, const double *w_a, int jm_a, const double *lat_b,
+                      const double *w_b, int jm_b, double *lat, double *lat_weight, int *jm_out);
+
 /* ---- ensemble batches: many members of uneven truncation and level count
  *      transformed by one hipGraph launch.  There is no SPML counterpart (SPML
  *      holds one global plan); this is the batched form of wa_xya/xya_wa for

@@ -79,5 +79,64 @@ def test_fortran_module_compiles_against_the_abi():
     if not shutil.which("amdflang") and not os.path.exists("/opt/rocm/bin/amdflang"):
         pytest.skip("amdflang not available")
     libdir = os.path.dirname(shtamd.LIB_PATH)
-    assert os.path.exists(os.path.join(libdir, "wa_module.mod"))
-    assert os.path.exists(os.path.join(libdir, "test_wa_module"))
+    for mod in ("wa_module", "l_module"):
+        assert os.path.exists(os.path.join(libdir, f"{mod}.mod"))
+        assert os.path.exists(os.path.join(libdir, f"test_{mod}"))
+
+
+def test_zonal_descriptor_validation_without_gpu():
+    h = ctypes.c_void_p()
+    L = shtamd.lib()
+    for args in ((10, 31, 1), (10, 0, 1), (33, 32, 1), (-1, 32, 1), (10, 32, 0)):
+        assert L.sht_zonal_create(*args, ctypes.byref(h)) == shtamd.SHT_ERR_INVALID, args
+        assert not h.value
+    assert L.sht_l_y(None, None, None, 1) == shtamd.SHT_ERR_INVALID
+    L.sht_zonal_destroy(None)
+
+
+# ---- gmapgen's cell edges and surface exchange grid (host utilities of the ABI)
+import numpy as np  # noqa: E402
+
+import oracle  # noqa: E402
+
+
+@pytest.mark.parametrize("jm", [2, 32, 64, 128, 1024])
+def test_lat_cell_edges_close_and_contain_nodes(jm):
+    """tool/gmapgen/gmapgen_main.f90:364-368 (and EBM_main.f90:200-204): the
+    edges start at -pi/2, close at +pi/2 because the weights sum to 2, each
+    Gauss node lies inside its cell, and each cell's area sin(e_j) - sin(e_j-1)
+    is the node's weight."""
+    mu, w, cl = oracle.gauss(jm)
+    e = shtamd.lat_cell_edges(w)
+    assert e.shape == (jm + 1,) and e[0] == -np.pi / 2 and e[-1] == np.pi / 2
+    lat = np.arctan2(mu, cl)
+    assert np.all(lat > e[:-1]) and np.all(lat < e[1:])
+    # the last interior edge agrees with the closing edge's sin to rounding
+    assert abs(np.sin(e[-2]) + w[-1] - 1.0) < 1e-12
+    assert np.abs(np.diff(np.sin(e))[:-1] - w[:-1]).max() < 1e-13
+    # the oracle's form of the same edges (cumulative weights in mu)
+    assert np.abs(np.sin(e[1:-1]) - (-1.0 + np.cumsum(w)[:-1])).max() < 1e-13
+
+
+@pytest.mark.parametrize("ja,jb", [(32, 64), (64, 32), (32, 48), (128, 64), (16, 16)])
+def test_exchange_grid(ja, jb):
+    """gmapgen_main.f90:336-387: the merged grid's bands are the union of both
+    grids' cells (sorted, zero-area bands dropped), weights sum to 2, each
+    latitude is its band's midpoint; equal latitude counts give grid A."""
+    mua, wa, cla = oracle.gauss(ja)
+    mub, wb, clb = oracle.gauss(jb)
+    la, lb = np.arctan2(mua, cla), np.arctan2(mub, clb)
+    lat, w = shtamd.exchange_grid(la, wa, lb, wb)
+    if ja == jb:
+        assert np.array_equal(lat, la) and np.array_equal(w, wa)
+        return
+    ea, eb = shtamd.lat_cell_edges(wa), shtamd.lat_cell_edges(wb)
+    merged = np.unique(np.round(np.concatenate([ea, eb]), 12))
+    assert lat.size == merged.size - 1
+    assert abs(w.sum() - 2.0) < 1e-12 and np.all(w > 1e-12)
+    assert np.all(np.diff(lat) > 0)
+    assert np.abs(lat - 0.5 * (merged[:-1] + merged[1:])).max() < 1e-11
+    assert np.abs(w - np.diff(np.sin(merged))).max() < 1e-11
+    # every band lies inside one cell of A and one of B
+    for x in lat:
+        assert np.searchsorted(ea, x) >= 1 and np.searchsorted(eb, x) >= 1

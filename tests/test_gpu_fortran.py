@@ -1,6 +1,8 @@
-"""The Fortran wa_module drop-in (ISO_C_BINDING over the C ABI), compiled with
-amdflang by the build, run as a program on the GPU: SPML names and array
-shapes, closed-form harmonics and gradients, round trip, global mean."""
+"""The Fortran drop-in modules (ISO_C_BINDING over the C ABI), compiled with
+amdflang by the build, run as programs on the GPU: wa_module (SPML names and
+array shapes, closed-form harmonics and gradients, round trip, global mean,
+divergence, Laplacian) and l_module at the energy-balance model's
+l_initial(1024, 1024) (calc_Hn_exact, round trip)."""
 import os
 import subprocess
 
@@ -9,12 +11,15 @@ import pytest
 import shtamd
 
 pytestmark = pytest.mark.gpu
-EXE = os.path.join(os.path.dirname(shtamd.LIB_PATH), "test_wa_module")
+LIBDIR = os.path.dirname(shtamd.LIB_PATH)
 
 
-@pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran drop-in not built (no amdflang)")
-def test_fortran_wa_module_dropin():
-    r = subprocess.run([EXE], capture_output=True, text=True, timeout=120)
+@pytest.mark.parametrize("module", ["wa_module", "l_module"])
+def test_fortran_dropin(module):
+    exe = os.path.join(LIBDIR, f"test_{module}")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran drop-in not built (no amdflang)")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
     assert "PASS" in r.stdout

@@ -247,6 +247,18 @@ def test_error_codes():
     with pytest.raises(shtamd.ShtError) as e:
         p.spec_to_grad(s, None, None, g, 2)  # GradLat without SHT_FLAG_GRAD
     assert e.value.code == shtamd.SHT_ERR_INVALID
+    # the binding refuses wrong dtypes, short buffers and mis-shaped host arrays
+    # before any pointer reaches the library
+    with pytest.raises(ValueError):
+        p.spec_to_grid(s.float(), g, 2)
+    with pytest.raises(ValueError):
+        p.spec_to_grid(s, g[:1], 2)
+    with pytest.raises(ValueError):
+        p.grid_to_spec(g.cpu(), s, 2)
+    with pytest.raises(ValueError):
+        p.wa_lapla_wa(np.zeros(22 * 22))  # 1-D: no level axis
+    with pytest.raises(ValueError):
+        p.xya_wa(np.zeros((2, 22 * 22 + 1)))
     p.close()
 
 

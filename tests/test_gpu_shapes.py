@@ -187,6 +187,15 @@ def test_shape_argument_checks():
     p.set_shapes(4, shtamd.SHAPE_AUTO, shtamd.SHAPE_AUTO)
     assert p.get_shapes(4) == p.get_shapes(0)
     p.close()
+    # an AUTO half of an entry resolves at call time, for the entry's own field
+    # count: the plan default if set, else the built-in choice for that count
+    q = shtamd.Plan(21, 64, 32, 40)
+    q.set_shapes(36, shtamd.SHAPE_AUTO, shtamd.SHAPE_KSPLIT)
+    assert q.get_shapes(36) == (shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT)
+    q.set_shapes(0, shtamd.SHAPE_TILE, shtamd.SHAPE_AUTO)
+    assert q.get_shapes(36) == (shtamd.SHAPE_TILE, shtamd.SHAPE_KSPLIT)
+    assert q.get_shapes(3) == (shtamd.SHAPE_TILE, shtamd.SHAPE_STAGED)
+    q.close()
     m = shtamd.Plan(21, 64, 32, 8, precision=shtamd.PREC_SPLIT_BF16X6)
     with pytest.raises(shtamd.ShtError):
         m.set_shapes(0, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_TILE_SPLIT)
