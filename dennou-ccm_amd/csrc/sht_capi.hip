@@ -15,28 +15,10 @@ int fail(int code, const std::string &msg) {
   return code;
 }
 
-static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
-
 static int validate_desc(const sht_desc *d) {
-  if (!d) return fail(SHT_ERR_INVALID, "sht_plan_create: desc is NULL");
-  if (d->nmax < 0 || d->nmax > 255)
-    return fail(SHT_ERR_INVALID, "sht_plan_create: nmax must be in [0, 255]");
-  if (d->jm < 2 || (d->jm & 1) || d->jm > 512)
-    return fail(SHT_ERR_INVALID, "sht_plan_create: jm must be even and in [2, 512]");
-  if (d->jm <= d->nmax)
-    return fail(SHT_ERR_INVALID, "sht_plan_create: jm must exceed nmax (exact quadrature)");
-  if (d->im < 2 * d->nmax + 2)
-    return fail(SHT_ERR_INVALID, "sht_plan_create: im must be >= 2*nmax + 2");
-  if (!is_pow2(d->im) || d->im < 16 || d->im > 2048)
-    return fail(SHT_ERR_UNSUPPORTED, "sht_plan_create: im must be a power of two in [16, 2048]");
-  if (d->kmax < 1) return fail(SHT_ERR_INVALID, "sht_plan_create: kmax must be >= 1");
-  if (d->precision < SHT_PREC_FP64 || d->precision > SHT_PREC_SPLIT_BF16X6)
-    return fail(SHT_ERR_INVALID, "sht_plan_create: unknown precision");
-  if (d->flags & ~SHT_FLAG_GRAD) return fail(SHT_ERR_INVALID, "sht_plan_create: unknown flags");
-  const size_t four = (size_t)(d->jm / 2) * (d->nmax + 1) * 4 * (size_t)d->kmax;
-  if (four > ((size_t)1 << 34))
-    return fail(SHT_ERR_INVALID, "sht_plan_create: kmax too large for the Fourier scratch");
-  return SHT_OK;
+  std::string msg;
+  const int rc = host::check_plan_desc(d, msg);
+  return rc ? fail(rc, msg) : SHT_OK;
 }
 
 static void release(Plan &p) {
@@ -208,6 +190,7 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   p.prec = desc->precision;
   p.flags = desc->flags;
   p.maxke = ceil_half(p.N + 1);
+  p.alloc_fail = host::injected_count("SHT_INJECT_PLAN_FAILURE");
   hipError_t e = hipGetDevice(&p.device);
   if (e != hipSuccess) {
     delete h;
@@ -219,9 +202,8 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   if (!rc) {
     // [P2][nfg][N+1][2][2G]: field groups of G = 2048/im are allocated whole
     const size_t four = (size_t)p.P2 * (p.N + 1) * 4 * (size_t)roundup(p.kmax, fft_group(p.im));
-    e = hipMalloc(&p.four, sizeof(double) * four);
-    if (e == hipSuccess && (p.flags & SHT_FLAG_GRAD)) e = hipMalloc(&p.four2, sizeof(double) * four);
-    if (e != hipSuccess) rc = fail(SHT_ERR_NOMEM, "sht_plan_create: Fourier scratch allocation failed");
+    rc = plan_alloc(p, reinterpret_cast<void **>(&p.four), sizeof(double) * four);
+    if (!rc && (p.flags & SHT_FLAG_GRAD)) rc = plan_alloc(p, reinterpret_cast<void **>(&p.four2), sizeof(double) * four);
   }
   if (rc) {
     std::string keep = g_err;
@@ -289,19 +271,13 @@ int sht_plan_stage_times(sht_plan *h, double *total_ms, long long *launches) {
 }
 
 int sht_l_nm(int n, int m) {
-  if (n < 0 || m < -n || m > n) return fail(SHT_ERR_INVALID, "sht_l_nm: need 0 <= |m| <= n");
-  // l <= (n + 1)^2 must fit an int
-  if (n > 46339) return fail(SHT_ERR_INVALID, "sht_l_nm: n above 46339 overflows l");
-  return n * (n + 1) + m + 1;
+  const int l = host::l_of_nm(n, m);
+  if (l < 1) return fail(SHT_ERR_INVALID, "sht_l_nm: need 0 <= |m| <= n <= 46339");
+  return l;
 }
 
 int sht_nm_l(int l, int *n, int *m) {
-  if (l < 1 || !n || !m) return fail(SHT_ERR_INVALID, "sht_nm_l: need l >= 1");
-  long long nn = (long long)std::sqrt((double)(l - 1));  // n = floor(sqrt(l - 1)), corrected below
-  while (nn > 0 && nn * nn > (long long)l - 1) --nn;
-  while ((nn + 1) * (nn + 1) <= (long long)l - 1) ++nn;
-  *n = (int)nn;
-  *m = (int)((long long)l - 1 - nn * (nn + 1));
+  if (!n || !m || !host::nm_of_l(l, *n, *m)) return fail(SHT_ERR_INVALID, "sht_nm_l: need l >= 1");
   return SHT_OK;
 }
 

@@ -13,6 +13,7 @@
 #pragma GCC visibility push(default)
 #include "../../include/sht.h"
 #pragma GCC visibility pop
+#include "sht_host.h"
 
 namespace sht {
 
@@ -81,8 +82,13 @@ struct Plan {
   int xcd_mlen[2] = {0, 0};
   // Legendre kernel shapes (SHT_SHAPE_*): the default for every field count,
   // and entries fixed by sht_plan_set_shapes / sht_plan_autotune for one count
+  // and the wavenumber classes of the split shape (host::Tuning: K loop split
+  // four ways for m < m4, two ways for m4 <= m < m2; -1 = built-in classes)
   int shape_inv = SHT_SHAPE_AUTO, shape_fwd = SHT_SHAPE_AUTO;
-  struct ShapeEntry { int nf, inv, fwd; };
+  struct ShapeEntry {
+    int nf;
+    host::Tuning t;
+  };
   std::vector<ShapeEntry> shapes;
   // scratch
   double *four = nullptr, *four2 = nullptr; // [P2][N+1][2][2*kmax]
@@ -97,7 +103,24 @@ struct Plan {
   struct TimedLaunch { int stage; hipEvent_t a, b; };
   std::vector<TimedLaunch> timed;
   std::vector<hipEvent_t> event_pool;
+  // SHT_INJECT_PLAN_FAILURE: the alloc_fail-th device allocation of this plan fails
+  int alloc_fail = 0, alloc_count = 0;
 };
+
+// Device allocation of plan state: SHT_ERR_NOMEM on failure, or when the
+// injected failure index is reached.
+inline int plan_alloc(Plan &p, void **ptr, size_t bytes) {
+  if (p.alloc_fail && ++p.alloc_count == p.alloc_fail)
+    return fail(SHT_ERR_NOMEM, "sht_plan_create: injected allocation failure (SHT_INJECT_PLAN_FAILURE)");
+  const hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return fail(SHT_ERR_NOMEM, std::string("sht_plan_create: hipMalloc: ") + hipGetErrorString(e));
+  return SHT_OK;
+}
+#define SHT_PLAN_ALLOC(p, ptr, bytes)                                              \
+  do {                                                                             \
+    const int rc_ = ::sht::plan_alloc((p), reinterpret_cast<void **>(ptr), (bytes)); \
+    if (rc_) return rc_;                                                           \
+  } while (0)
 
 __host__ __device__ inline int roundup(int a, int b) { return (a + b - 1) / b * b; }
 
@@ -190,10 +213,20 @@ int launch_fft_inv(const Plan &p, const void *four, const void *four2, bool f32,
 // Legendre stage, FP64 MFMA (sht_legendre.hip).  Shapes resolve through
 // resolve_shape: an entry for nf, else the plan default, else the built-in one.
 int resolve_shape(const Plan &p, int nf, int dir);  // dir 0 inverse, 1 forward
+// Wavenumber classes of the split shape (SHT_SHAPE_TILE_SPLIT), numbered
+// heaviest first: the K loop runs split four ways for m < m4, two ways for
+// m4 <= m < m2, unsplit for m >= m2.
+struct KSplit {
+  int m4, m2;
+};
+// The classes a call with nf fields uses: an entry's, else the built-in ones.
+KSplit resolve_classes(const Plan &p, int nf, int dir);
+KSplit default_classes(const Plan &p, int dir);
 int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, double *four,
-                             double *four2, bool value_table, bool deriv_table, int nf, hipStream_t s);
+                             double *four2, bool value_table, bool deriv_table, int nf, hipStream_t s,
+                             KSplit cls = KSplit{-1, -1});
 int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
-                             hipStream_t s);
+                             hipStream_t s, KSplit cls = KSplit{-1, -1});
 int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
                        bool value_table, bool deriv_table, int nf, hipStream_t s);
 int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s);

@@ -349,10 +349,6 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_wave_kernel(
 // [m4, m2) ks = 2, [m2, N] ks = 1 (SHT_SHAPE_TILE: m4 = m2 = 0, no split;
 // SHT_SHAPE_TILE_SPLIT: thresholds on the K length).  Partial sums of a split
 // meet in LDS in FP64.
-struct KSplit {
-  int m4, m2;
-};
-
 template <int RT, int D, bool NT>
 __global__ __launch_bounds__(256) void leg_inv_f64_tile_kernel(
     const double *__restrict__ spec, double *__restrict__ four, double *__restrict__ four2,
@@ -1051,11 +1047,35 @@ int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStrea
 
 }  // namespace
 
+KSplit default_classes(const Plan &p, int dir) {
+  if (dir == 0) {
+    // halve the K loop of the wavenumbers with at least 70 % of the heaviest
+    // one's K steps (the waves that set the critical path); no four-way
+    // split.  Measured at T85: 97 fields 10.9 us (thresholds 6/3 gave 15.4,
+    // TILE 13.5); T42, 388 fields 7.2 us (9.1, TILE 12.1).
+    const int nsmax = roundup(ceil_half(p.N + 1), 4) / 4;
+    return ksplit_inv(p, 1 << 30, (7 * nsmax + 9) / 10);
+  }
+  // halve the pair loop of the wavenumbers with the most row tiles only
+  // (thresholds 2/1, which split every m, were slower everywhere)
+  return ksplit_fwd(p, 1 << 30, roundup(ceil_half(p.N + 1), 16) / 16);
+}
+
+KSplit resolve_classes(const Plan &p, int nf, int dir) {
+  for (const Plan::ShapeEntry &e : p.shapes)
+    if (e.nf == nf) {
+      const int m4 = dir ? e.t.fwd_m4 : e.t.inv_m4, m2 = dir ? e.t.fwd_m2 : e.t.inv_m2;
+      if (m4 >= 0 && m2 >= m4 && m2 <= p.N + 1) return KSplit{m4, m2};
+      break;
+    }
+  return default_classes(p, dir);
+}
+
 int resolve_shape(const Plan &p, int nf, int dir) {
   // an entry for nf, unless its half for this direction is AUTO
   for (const Plan::ShapeEntry &e : p.shapes)
     if (e.nf == nf) {
-      const int s = dir ? e.fwd : e.inv;
+      const int s = dir ? e.t.fwd : e.t.inv;
       if (s != SHT_SHAPE_AUTO) return s;
       break;
     }
@@ -1076,7 +1096,7 @@ int resolve_shape(const Plan &p, int nf, int dir) {
 
 int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, double *four,
                              double *four2, bool value_table, bool deriv_table, int nf,
-                             hipStream_t s) {
+                             hipStream_t s, KSplit cls) {
   if (!value_table && !deriv_table) return SHT_OK;
   const int tab0 = value_table ? 0 : 1;
   const int ntab = (value_table && deriv_table) ? 2 : 1;
@@ -1091,12 +1111,8 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
     const int rtiles = p.P2p / 16;
     int rt = 4;
     while (rtiles % rt) --rt;
-    // TILE_SPLIT: halve the K loop of the wavenumbers with at least 70 % of the
-    // heaviest one's K steps (the waves that set the critical path); no
-    // four-way split.  Measured at T85: 97 fields 10.9 us (thresholds 6/3 gave
-    // 15.4, TILE 13.5); T42, 388 fields 7.2 us (9.1, TILE 12.1).
-    const int nsmax = roundup(ceil_half(p.N + 1), 4) / 4;
-    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_inv(p, 1 << 30, (7 * nsmax + 9) / 10);
+    // TILE_SPLIT: the given wavenumber classes, else the built-in ones
+    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : (cls.m2 >= 0 ? cls : default_classes(p, 0));
     switch (rt) {
       case 4: return inv_tile_launch<4, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
       case 3: return inv_tile_launch<3, 2>(p, k, spec, four, four2, tab0, ntab, nf, s);
@@ -1120,7 +1136,7 @@ int launch_leg_inv_f64_shape(const Plan &p, int shape, const double *spec, doubl
 }
 
 int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, double *spec, int nf,
-                             hipStream_t s) {
+                             hipStream_t s, KSplit cls) {
   if (shape == SHT_SHAPE_WAVE) return fwd_wave_launch<4>(p, four, spec, nf, s);
   if (shape == SHT_SHAPE_STAGED && p.P2p == 128 && nf >= kHalvesMinFields)
     return fwd_halves_launch<2, 2>(p, four, spec, nf, s);
@@ -1128,10 +1144,8 @@ int launch_leg_fwd_f64_shape(const Plan &p, int shape, const double *four, doubl
     return p.P2p >= 128 ? fwd_staged_launch<1, 8>(p, four, spec, nf, s) : fwd_staged_launch<1, 4>(p, four, spec, nf, s);
   const int rt = (ceil_half(p.N + 1) + 15) / 16;
   if (shape == SHT_SHAPE_TILE || shape == SHT_SHAPE_TILE_SPLIT) {
-    // TILE_SPLIT: halve the pair loop of the wavenumbers with the most row
-    // tiles only (thresholds 2/1, which split every m, were slower everywhere)
-    const int nrtmax = roundup(ceil_half(p.N + 1), 16) / 16;
-    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : ksplit_fwd(p, 1 << 30, nrtmax);
+    // TILE_SPLIT: the given wavenumber classes, else the built-in ones
+    const KSplit k = shape == SHT_SHAPE_TILE ? KSplit{0, 0} : (cls.m2 >= 0 ? cls : default_classes(p, 1));
     switch (rt) {
       case 1: return fwd_tile_launch<1, 2>(p, k, four, spec, nf, s);
       case 2: return fwd_tile_launch<2, 2>(p, k, four, spec, nf, s);
@@ -1175,11 +1189,11 @@ int launch_leg_fwd_f64_table(const Plan &p, const double *table, const double *f
 int launch_leg_inv_f64(const Plan &p, const double *spec, double *four, double *four2,
                        bool value_table, bool deriv_table, int nf, hipStream_t s) {
   return launch_leg_inv_f64_shape(p, resolve_shape(p, nf, 0), spec, four, four2, value_table,
-                                  deriv_table, nf, s);
+                                  deriv_table, nf, s, resolve_classes(p, nf, 0));
 }
 
 int launch_leg_fwd_f64(const Plan &p, const double *four, double *spec, int nf, hipStream_t s) {
-  return launch_leg_fwd_f64_shape(p, resolve_shape(p, nf, 1), four, spec, nf, s);
+  return launch_leg_fwd_f64_shape(p, resolve_shape(p, nf, 1), four, spec, nf, s, resolve_classes(p, nf, 1));
 }
 
 }  // namespace sht

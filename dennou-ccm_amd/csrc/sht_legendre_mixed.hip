@@ -705,12 +705,12 @@ int build_mixed_mode(Plan &p) {
     return fail(SHT_ERR_UNSUPPORTED, "mixed-precision table exceeds 2^31 entries");
   p.n_linv = ti;
   p.n_lfwd = tf;
-  SHT_HIP(hipMalloc(&p.Linv, sizeof(uint16_t) * ti));
-  SHT_HIP(hipMalloc(&p.Lfwd, sizeof(uint16_t) * tf));
-  SHT_HIP(hipMalloc(&p.loff_inv, sizeof(int) * (N + 1)));
-  SHT_HIP(hipMalloc(&p.loff_fwd, sizeof(int) * (N + 1)));
-  SHT_HIP(hipMalloc(&p.lexp_inv, sizeof(int) * 2 * (N + 1)));
-  SHT_HIP(hipMalloc(&p.lexp_fwd, sizeof(int) * 2 * (N + 1)));
+  SHT_PLAN_ALLOC(p, &p.Linv, sizeof(uint16_t) * ti);
+  SHT_PLAN_ALLOC(p, &p.Lfwd, sizeof(uint16_t) * tf);
+  SHT_PLAN_ALLOC(p, &p.loff_inv, sizeof(int) * (N + 1));
+  SHT_PLAN_ALLOC(p, &p.loff_fwd, sizeof(int) * (N + 1));
+  SHT_PLAN_ALLOC(p, &p.lexp_inv, sizeof(int) * 2 * (N + 1));
+  SHT_PLAN_ALLOC(p, &p.lexp_fwd, sizeof(int) * 2 * (N + 1));
   SHT_HIP(hipMemcpy(p.loff_inv, li.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   SHT_HIP(hipMemcpy(p.loff_fwd, lf.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   SHT_HIP(hipMemset(p.Linv, 0, sizeof(uint16_t) * ti));

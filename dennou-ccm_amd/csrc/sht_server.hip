@@ -12,6 +12,7 @@
 //      -> event.  A batch of one request runs in place (no gather/scatter);
 //   3. with nothing to do it sleeps on a condition variable (woken by submit)
 //      or, while batches run, polls their events every few microseconds.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -75,14 +76,11 @@ __global__ void grid_move(ReqTable t, double *batch, int npt, int nft, int gathe
   else q.dst[ri] = batch[e];
 }
 
-struct Request {
-  long long ticket;
-  int g, nf, dir;
-  const double *in;
-  double *out;
-  size_t bytes;
-  Clock::time_point t_submit;
-};
+using Request = host::Pending;
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(Clock::now().time_since_epoch()).count();
+}
 
 struct Slot {
   hipStream_t s = nullptr;
@@ -103,14 +101,14 @@ struct sht_server {
   std::thread worker;
   std::mutex mu;
   std::condition_variable cv;
-  std::deque<sht::Request> pending;
+  sht::host::Admission adm{0, 1, 1};                       // pending requests, budget, counters
   std::vector<std::pair<long long, double>> completed;
-  size_t bytes_used = 0;
   int inflight_batches = 0;
   bool stop = false;
-  long long next_ticket = 1;
-  long long n_req = 0, n_fields = 0, n_batches = 0, n_refused = 0;
-  std::string worker_error;
+  int fail_batch = 0;     // SHT_INJECT_BATCH_FAILURE: this batch (1-based) fails before launch
+  long long launched = 0;
+  std::string worker_error;   // a device fault: sticky, every later call fails
+  std::string batch_error;    // the last failed batch's reason (its requests report -1)
 };
 
 namespace sht {
@@ -124,12 +122,13 @@ size_t spec_doubles(const sht_server_desc &d, int g) {
 // Enqueues one batch on slot sl; returns a HIP error string on failure.
 std::string launch_batch(sht_server *S, Slot &sl, std::vector<Request> &batch) {
   const sht_server_desc &d = S->d;
-  const int g = batch[0].g, dir = batch[0].dir;
+  const int g = batch[0].grid, dir = batch[0].dir;
   const int K = (d.nmax[g] + 1) * (d.nmax[g] + 2) / 2;
   const int npt = d.im[g] * d.jm[g];
   sht_plan *plan = sl.plans[g];
   int nft = 0;
   for (auto &r : batch) nft += r.nf;
+  if (S->fail_batch && ++S->launched == S->fail_batch) return "injected batch failure (SHT_INJECT_BATCH_FAILURE)";
   int rc;
   if (batch.size() == 1) {  // in place
     const Request &r = batch[0];
@@ -177,16 +176,11 @@ void worker_loop(sht_server *S) {
       if (!sl.busy) continue;
       hipError_t q = hipEventQuery(sl.done);
       if (q == hipErrorNotReady) continue;
-      const auto now = Clock::now();
+      const double now = now_ms();
       std::lock_guard<std::mutex> lk(S->mu);
       if (q != hipSuccess && S->worker_error.empty()) S->worker_error = hipGetErrorString(q);
-      for (auto &r : sl.inflight) {
-        const double ms = std::chrono::duration<double, std::milli>(now - r.t_submit).count();
-        S->completed.emplace_back(r.ticket, q == hipSuccess ? ms : -1.0);
-        S->bytes_used -= r.bytes;
-        S->n_req += 1;
-        S->n_fields += r.nf;
-      }
+      for (auto &r : sl.inflight) S->completed.emplace_back(r.ticket, q == hipSuccess ? now - r.t_submit : -1.0);
+      S->adm.finish(sl.inflight, q == hipSuccess);
       sl.inflight.clear();
       sl.busy = false;
       S->inflight_batches -= 1;
@@ -198,30 +192,19 @@ void worker_loop(sht_server *S) {
       std::vector<Request> batch;
       {
         std::lock_guard<std::mutex> lk(S->mu);
-        if (S->pending.empty()) break;
-        const int g = S->pending.front().g, dir = S->pending.front().dir;
-        int nf = 0;
-        for (auto it = S->pending.begin(); it != S->pending.end();) {
-          if (it->g == g && it->dir == dir && nf + it->nf <= d.max_fields &&
-              (int)batch.size() < kMaxBatchReq) {
-            nf += it->nf;
-            batch.push_back(*it);
-            it = S->pending.erase(it);
-          } else {
-            ++it;
-          }
-        }
+        if (S->adm.empty()) break;
+        batch = S->adm.take_batch();
         S->inflight_batches += 1;
-        S->n_batches += 1;
       }
       std::string err = launch_batch(S, sl, batch);
       if (!err.empty()) {
+        // this batch's requests fail (latency -1); the others go on, since
+        // nothing of it reached the device
+        (void)hipGetLastError();
         std::lock_guard<std::mutex> lk(S->mu);
-        if (S->worker_error.empty()) S->worker_error = err;
-        for (auto &r : batch) {
-          S->completed.emplace_back(r.ticket, -1.0);
-          S->bytes_used -= r.bytes;
-        }
+        S->batch_error = err;
+        for (auto &r : batch) S->completed.emplace_back(r.ticket, -1.0);
+        S->adm.finish(batch, false);
         S->inflight_batches -= 1;
       }
       progressed = true;
@@ -229,9 +212,8 @@ void worker_loop(sht_server *S) {
     if (progressed) continue;
     std::unique_lock<std::mutex> lk(S->mu);
     if (S->inflight_batches == 0) {
-      if (S->stop && S->pending.empty()) return;
-      S->cv.wait_for(lk, std::chrono::milliseconds(2),
-                     [&] { return S->stop || !S->pending.empty(); });
+      if (S->stop && S->adm.empty()) return;
+      S->cv.wait_for(lk, std::chrono::milliseconds(2), [&] { return S->stop || !S->adm.empty(); });
     } else {
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(5));
@@ -273,6 +255,8 @@ int sht_server_create(const sht_server_desc *desc, sht_server **out) {
   sht_server *S = new (std::nothrow) sht_server;
   if (!S) return fail(SHT_ERR_NOMEM, "sht_server_create: host allocation failed");
   S->d = *desc;
+  S->adm = host::Admission(desc->mem_budget, desc->max_fields, kMaxBatchReq);
+  S->fail_batch = host::injected_count("SHT_INJECT_BATCH_FAILURE");
   if (hipGetDevice(&S->device) != hipSuccess) {
     delete S;
     return fail(SHT_ERR_DEVICE, "sht_server_create: hipGetDevice failed");
@@ -337,14 +321,8 @@ int sht_server_submit(sht_server *S, const sht_request *r, long long *ticket) {
   {
     std::lock_guard<std::mutex> lk(S->mu);
     if (!S->worker_error.empty()) return fail(SHT_ERR_DEVICE, "sht_server_submit: " + S->worker_error);
-    if (S->bytes_used + bytes > S->d.mem_budget) {
-      S->n_refused += 1;
+    if (S->adm.submit(g, r->nfield, r->direction, r->in, r->out, bytes, now_ms(), ticket) != SHT_OK)
       return fail(SHT_ERR_BUSY, "sht_server_submit: memory budget exhausted");
-    }
-    S->bytes_used += bytes;
-    Request q{S->next_ticket++, g, r->nfield, r->direction, r->in, r->out, bytes, Clock::now()};
-    S->pending.push_back(q);
-    *ticket = q.ticket;
   }
   S->cv.notify_one();
   return SHT_OK;
@@ -371,7 +349,7 @@ int sht_server_drain(sht_server *S) {
     {
       std::lock_guard<std::mutex> lk(S->mu);
       if (!S->worker_error.empty()) return fail(SHT_ERR_DEVICE, "sht_server: " + S->worker_error);
-      if (S->pending.empty() && S->inflight_batches == 0) return SHT_OK;
+      if (S->adm.empty() && S->inflight_batches == 0) return SHT_OK;
     }
     std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
@@ -382,10 +360,24 @@ int sht_server_stats(const sht_server *S, long long *requests, long long *fields
   if (!S) return fail(SHT_ERR_INVALID, "sht_server_stats: server is NULL");
   sht_server *M = const_cast<sht_server *>(S);
   std::lock_guard<std::mutex> lk(M->mu);
-  if (requests) *requests = S->n_req;
-  if (fields) *fields = S->n_fields;
-  if (batches) *batches = S->n_batches;
-  if (refused) *refused = S->n_refused;
+  if (requests) *requests = S->adm.completed();
+  if (fields) *fields = S->adm.completed_fields();
+  if (batches) *batches = S->adm.batches();
+  if (refused) *refused = S->adm.refused();
+  return SHT_OK;
+}
+
+int sht_server_failures(const sht_server *S, long long *failed, char *reason, int reason_len) {
+  if (!S) return fail(SHT_ERR_INVALID, "sht_server_failures: server is NULL");
+  sht_server *M = const_cast<sht_server *>(S);
+  std::lock_guard<std::mutex> lk(M->mu);
+  if (failed) *failed = S->adm.failed();
+  if (reason && reason_len > 0) {
+    const std::string &e = S->worker_error.empty() ? S->batch_error : S->worker_error;
+    const size_t n = std::min(e.size(), (size_t)reason_len - 1);
+    e.copy(reason, n);
+    reason[n] = '\0';
+  }
   return SHT_OK;
 }
 

@@ -165,18 +165,18 @@ int build_tables(Plan &p) {
     return fail(SHT_ERR_UNSUPPORTED, "Legendre table exceeds 2^31 entries");
   p.n_inv = ti;
   p.n_fwd = tf;
-  SHT_HIP(hipMalloc(&p.mu, sizeof(double) * p.jm));
-  SHT_HIP(hipMalloc(&p.w, sizeof(double) * p.jm));
-  SHT_HIP(hipMalloc(&p.coslat, sizeof(double) * p.jm));
-  SHT_HIP(hipMalloc(&p.twid, sizeof(double) * 2 * p.im));
-  SHT_HIP(hipMalloc(&p.zero, sizeof(double) * 64));
+  SHT_PLAN_ALLOC(p, &p.mu, sizeof(double) * p.jm);
+  SHT_PLAN_ALLOC(p, &p.w, sizeof(double) * p.jm);
+  SHT_PLAN_ALLOC(p, &p.coslat, sizeof(double) * p.jm);
+  SHT_PLAN_ALLOC(p, &p.twid, sizeof(double) * 2 * p.im);
+  SHT_PLAN_ALLOC(p, &p.zero, sizeof(double) * 64);
   SHT_HIP(hipMemset(p.zero, 0, sizeof(double) * 64));
-  SHT_HIP(hipMalloc(&p.Pinv, sizeof(double) * ti));
-  SHT_HIP(hipMalloc(&p.Pfwd, sizeof(double) * tf));
-  if (p.flags & SHT_FLAG_GRAD) SHT_HIP(hipMalloc(&p.Hinv, sizeof(double) * ti));
-  if (p.flags & SHT_FLAG_GRAD) SHT_HIP(hipMalloc(&p.Hfwd, sizeof(double) * tf));
-  SHT_HIP(hipMalloc(&p.poff_inv, sizeof(int) * (N + 1)));
-  SHT_HIP(hipMalloc(&p.poff_fwd, sizeof(int) * (N + 1)));
+  SHT_PLAN_ALLOC(p, &p.Pinv, sizeof(double) * ti);
+  SHT_PLAN_ALLOC(p, &p.Pfwd, sizeof(double) * tf);
+  if (p.flags & SHT_FLAG_GRAD) SHT_PLAN_ALLOC(p, &p.Hinv, sizeof(double) * ti);
+  if (p.flags & SHT_FLAG_GRAD) SHT_PLAN_ALLOC(p, &p.Hfwd, sizeof(double) * tf);
+  SHT_PLAN_ALLOC(p, &p.poff_inv, sizeof(int) * (N + 1));
+  SHT_PLAN_ALLOC(p, &p.poff_fwd, sizeof(int) * (N + 1));
   SHT_HIP(hipMemcpy(p.poff_inv, p.h_poff_inv.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   SHT_HIP(hipMemcpy(p.poff_fwd, p.h_poff_fwd.data(), sizeof(int) * (N + 1), hipMemcpyHostToDevice));
   SHT_HIP(hipMemset(p.Pinv, 0, sizeof(double) * ti));
@@ -187,7 +187,7 @@ int build_tables(Plan &p) {
   for (int m = 0; m <= N; ++m)
     for (int kt = 0; kt < roundup(ceil_half(N - m + 1), 16) / 16; ++kt) units.push_back(m << 8 | kt);
   p.n_fwd_units = (int)units.size();
-  SHT_HIP(hipMalloc(&p.fwd_units, sizeof(int) * units.size()));
+  SHT_PLAN_ALLOC(p, &p.fwd_units, sizeof(int) * units.size());
   SHT_HIP(hipMemcpy(p.fwd_units, units.data(), sizeof(int) * units.size(), hipMemcpyHostToDevice));
   for (int dir = 0; dir < 2; ++dir) {
     std::vector<int> work(N + 1), order(N + 1);
@@ -212,7 +212,7 @@ int build_tables(Plan &p) {
     for (int x = 0; x < 8; ++x)
       for (size_t j = 0; j < lists[x].size(); ++j) flat[x * len + j] = lists[x][j];
     p.xcd_mlen[dir] = (int)len;
-    SHT_HIP(hipMalloc(&p.xcd_m[dir], sizeof(int) * flat.size()));
+    SHT_PLAN_ALLOC(p, &p.xcd_m[dir], sizeof(int) * flat.size());
     SHT_HIP(hipMemcpy(p.xcd_m[dir], flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice));
   }
 

@@ -223,12 +223,8 @@ extern "C" {
 int sht_zonal_create(int nmax, int jm, int kmax, sht_zonal **out) {
   if (!out) return fail(SHT_ERR_INVALID, "sht_zonal_create: out is NULL");
   *out = nullptr;
-  if (jm < 2 || (jm & 1) || jm > 8192)
-    return fail(SHT_ERR_INVALID, "sht_zonal_create: jm must be even and in [2, 8192]");
-  if (nmax < 0 || nmax > jm)
-    return fail(SHT_ERR_INVALID, "sht_zonal_create: nmax must be in [0, jm]");
-  if (kmax < 1 || (size_t)kmax * (size_t)(jm > nmax + 1 ? jm : nmax + 1) > ((size_t)1 << 31))
-    return fail(SHT_ERR_INVALID, "sht_zonal_create: kmax out of range");
+  std::string msg;
+  if (int rc = host::check_zonal_desc(nmax, jm, kmax, msg)) return fail(rc, msg);
   sht_zonal *z = new (std::nothrow) sht_zonal;
   if (!z) return fail(SHT_ERR_NOMEM, "sht_zonal_create: host allocation failed");
   z->N = nmax;

@@ -102,6 +102,7 @@ SIGNATURES = {
     "sht_server_poll": (_ci, [_vp, ctypes.POINTER(ctypes.c_longlong), _dp, _ci, ctypes.POINTER(_ci)]),
     "sht_server_drain": (_ci, [_vp]),
     "sht_server_stats": (_ci, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 4),
+    "sht_server_failures": (_ci, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_char_p, _ci]),
     "sht_plan_set_timing": (_ci, [_vp, _ci]),
     "sht_plan_stage_times": (_ci, [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]),
     "sht_remap_create": (_ci, [ctypes.POINTER(_ci), ctypes.POINTER(_ci), _dp, _ci, _ci, _ci,
@@ -111,6 +112,8 @@ SIGNATURES = {
     "sht_plan_set_shapes": (_ci, [_vp, _ci, _ci, _ci]),
     "sht_plan_get_shapes": (_ci, [_vp, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_plan_autotune": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _dp]),
+    "sht_plan_set_classes": (_ci, [_vp, _ci, _ci, _ci, _ci]),
+    "sht_plan_get_classes": (_ci, [_vp, _ci, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_zonal_create": (_ci, [_ci, _ci, _ci, ctypes.POINTER(_vp)]),
     "sht_zonal_destroy": (None, [_vp]),
     "sht_zonal_grid": (_ci, [_vp, _dp, _dp]),
@@ -280,6 +283,15 @@ class Plan:
     def get_shapes(self, nfield: int):
         a, b = ctypes.c_int(), ctypes.c_int()
         check(lib().sht_plan_get_shapes(self._h, nfield, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_classes(self, nfield: int, direction: int, m4: int, m2: int):
+        """Wavenumber classes of the split shape for calls with nfield fields."""
+        check(lib().sht_plan_set_classes(self._h, nfield, direction, m4, m2))
+
+    def get_classes(self, nfield: int, direction: int):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        check(lib().sht_plan_get_classes(self._h, nfield, direction, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
     def autotune(self, nfield: int, reps: int = 20, cache_path: str | None = None):
@@ -523,6 +535,13 @@ class Server:
         v = [ctypes.c_longlong() for _ in range(4)]
         check(lib().sht_server_stats(self._h, *[ctypes.byref(x) for x in v]))
         return dict(zip(("requests", "fields", "batches", "refused"), (x.value for x in v)))
+
+    def failures(self):
+        """(failed requests, reason of the last failure)"""
+        n = ctypes.c_longlong()
+        buf = ctypes.create_string_buffer(512)
+        check(lib().sht_server_failures(self._h, ctypes.byref(n), buf, 512))
+        return n.value, buf.value.decode()
 
 
 def spec_to_real(spec_c: np.ndarray) -> np.ndarray:

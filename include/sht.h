@@ -82,6 +82,9 @@ typedef struct {
  *      weights, Pt_n^m and derivative tables (FP64, computed on the GPU),
  *      FFT twiddles.  Uses the current HIP device. */
 int sht_plan_create(const sht_desc *desc, sht_plan **plan);
+/* (SHT_INJECT_PLAN_FAILURE=k in the environment makes the k-th device
+ * allocation of every sht_plan_create fail with SHT_ERR_NOMEM, for tests of
+ * the clean-up path; unset in production.) */
 void sht_plan_destroy(sht_plan *plan);
 const char *sht_last_error(void);
 int sht_abi_version(void);
@@ -254,6 +257,14 @@ int sht_server_drain(sht_server *server);
 /* Counters: completed requests, fields, batches launched, requests refused. */
 int sht_server_stats(const sht_server *server, long long *requests, long long *fields,
                      long long *batches, long long *refused);
+/* Failed requests and the reason of the last failure (reason may be NULL).  A
+ * batch that fails before it reaches the device (a launch error) fails only
+ * its own requests -- sht_server_poll reports them with latency -1 -- and the
+ * server goes on; a device fault seen on a batch's completion is sticky and
+ * fails every later call with SHT_ERR_DEVICE.  Environment variable
+ * SHT_INJECT_BATCH_FAILURE=k (read at sht_server_create) makes the k-th batch
+ * fail before launch, for tests of this path. */
+int sht_server_failures(const sht_server *server, long long *failed, char *reason, int reason_len);
 
 /* ---- coupler remap: the sparse interpolation Jcup applies to exchanged
  *      fields (common/interpolation_data_latlon_mod.f90:274-306,
@@ -313,12 +324,24 @@ enum {
 int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
 /* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
 int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);
+/* Wavenumber classes of SHT_SHAPE_TILE_SPLIT for calls with nf fields in one
+ * direction (SHT_DIR_INVERSE or SHT_DIR_FORWARD): wavenumbers are numbered
+ * heaviest first (m = 0 has the longest K loop), the loop runs split four
+ * ways for m < m4, two ways for m4 <= m < m2 and unsplit above; m4 = m2 = -1
+ * restores the built-in classes (two-way for the wavenumbers with at least
+ * 70 % of the heaviest one's work).  The split changes the order of the K
+ * sums, so results agree with the other shapes to rounding. */
+int sht_plan_set_classes(sht_plan *plan, int nf, int dir, int m4, int m2);
+int sht_plan_get_classes(const sht_plan *plan, int nf, int dir, int *m4, int *m2);
 /* Times every shape of both Legendre kernels at nf fields on the plan's device
- * (reps runs of the whole direction, Legendre with FFT, per shape in three
- * interleaved rounds, on a private stream; synchronizes it) and keeps the
- * fastest for calls with nf fields.  With cache_path not NULL, a JSON file
- * keyed by truncation, grid, nf and precision is consulted first and updated
- * after timing.  best_us (2 doubles, may be NULL) receives the winners' mean
+ * -- for FP64 plans also the split shape over a range of wavenumber classes
+ * (two-way class ending after 20 % to 70 % of the wavenumbers, with no, a
+ * quarter or a half four-way class) -- (reps runs of the whole direction,
+ * Legendre with FFT, per candidate in three interleaved rounds, on a private
+ * stream; synchronizes it) and keeps the fastest shape and classes for calls
+ * with nf fields.  With cache_path not NULL, a JSON file keyed by truncation,
+ * grid, nf and precision (value: the two shapes and the two class pairs) is
+ * consulted first and updated after timing.  best_us (2 doubles, may be NULL) receives the winners' mean
  * direction times in microseconds (0 when taken from the cache).  It uses the plan's
  * scratch: call it while no other call on the plan is in flight. */
 int sht_plan_autotune(sht_plan *plan, int nf, int reps, const char *cache_path, double *best_us);
