@@ -30,7 +30,7 @@ SOURCES = ["sht_capi.hip", "sht_tables.hip", "sht_fft.hip", "sht_legendre.hip",
 
 
 # host-only C++ (no HIP), compiled by the host compiler
-HOST_SOURCES = ["sht_host.cpp"]
+HOST_SOURCES = ["sht_host.cpp", "sht_restart.cpp"]
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
 

@@ -7,11 +7,9 @@
 
 namespace sht {
 
-static thread_local std::string g_err;
-
-void set_error(const std::string &msg) { g_err = msg; }
+void set_error(const std::string &msg) { host::set_error(msg); }
 int fail(int code, const std::string &msg) {
-  g_err = msg;
+  host::set_error(msg);
   return code;
 }
 
@@ -169,7 +167,7 @@ using namespace sht;
 
 extern "C" {
 
-const char *sht_last_error(void) { return g_err.c_str(); }
+const char *sht_last_error(void) { return host::last_error(); }
 int sht_abi_version(void) { return SHT_ABI_VERSION; }
 
 int sht_plan_create(const sht_desc *desc, sht_plan **out) {
@@ -206,10 +204,10 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
     if (!rc && (p.flags & SHT_FLAG_GRAD)) rc = plan_alloc(p, reinterpret_cast<void **>(&p.four2), sizeof(double) * four);
   }
   if (rc) {
-    std::string keep = g_err;
+    std::string keep = host::last_error();
     release(p);
     delete h;
-    g_err = keep;
+    host::set_error(keep);
     return rc;
   }
   *out = h;

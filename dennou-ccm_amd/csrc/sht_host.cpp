@@ -11,6 +11,11 @@
 namespace sht {
 namespace host {
 
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+const char *last_error() { return g_error.c_str(); }
+
 static bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 int check_plan_desc(const sht_desc *d, std::string &msg) {

@@ -19,6 +19,10 @@
 namespace sht {
 namespace host {
 
+// ---- the calling thread's last error text (sht_last_error)
+void set_error(const std::string &msg);
+const char *last_error();
+
 // ---- descriptors: SHT_OK, or an SHT_ERR_* code with the reason in msg
 int check_plan_desc(const sht_desc *d, std::string &msg);
 int check_zonal_desc(int nmax, int jm, int kmax, std::string &msg);
@@ -89,6 +93,15 @@ class Admission {
   long long next_ticket_ = 1, refused_ = 0, batches_ = 0, completed_ = 0, fields_ = 0, failed_ = 0;
   std::deque<Pending> pending_;
 };
+
+// ---- restart state (sht_restart_write / sht_restart_read): the ten grid
+// fields of the two leapfrog time levels in a netCDF classic file
+struct RestartDims {
+  int im, jm, km, nq;
+};
+int restart_write(const char *path, const RestartDims &d, double time, const double *lon, const double *lat,
+                  const double *sig, const double *const *fields, std::string &msg);
+int restart_read(const char *path, const RestartDims &d, double *time, double *const *fields, std::string &msg);
 
 // ---- test hook: the value of environment variable `name` as a count
 // (0 when unset or not a positive integer).  SHT_INJECT_PLAN_FAILURE=k makes

@@ -55,6 +55,12 @@ class _ServerDesc(ctypes.Structure):
                 ("precision", ctypes.c_int), ("mem_budget", ctypes.c_size_t)]
 
 
+class _RestartDesc(ctypes.Structure):
+    _fields_ = [("im", ctypes.c_int), ("jm", ctypes.c_int), ("km", ctypes.c_int), ("nq", ctypes.c_int),
+                ("time", ctypes.c_double), ("lon", ctypes.c_void_p), ("lat", ctypes.c_void_p),
+                ("sig", ctypes.c_void_p)]
+
+
 class _Request(ctypes.Structure):
     _fields_ = [("nmax", ctypes.c_int), ("nfield", ctypes.c_int), ("direction", ctypes.c_int),
                 ("inp", ctypes.c_void_p), ("out", ctypes.c_void_p)]
@@ -122,6 +128,8 @@ SIGNATURES = {
     "sht_l_y": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_y_l": (_ci, [_vp, _dp, _dp, _ci]),
     "sht_lat_cell_edges": (_ci, [_dp, _ci, _dp]),
+    "sht_restart_write": (_ci, [ctypes.c_char_p, ctypes.POINTER(_RestartDesc), ctypes.POINTER(_vp)]),
+    "sht_restart_read": (_ci, [ctypes.c_char_p, ctypes.POINTER(_RestartDesc), _dp, ctypes.POINTER(_vp)]),
     "sht_exchange_grid": (_ci, [_dp, _dp, _ci, _dp, _dp, _ci, _dp, _dp, ctypes.POINTER(_ci)]),
 }
 
@@ -415,6 +423,43 @@ def exchange_grid(lat_a, w_a, lat_b, w_b):
     check(lib().sht_exchange_grid(_host(la), _host(wa), wa.size, _host(lb), _host(wb), wb.size,
                                   _host(lat), _host(w), ctypes.byref(n)))
     return lat[:n.value].copy(), w[:n.value].copy()
+
+
+RESTART_FIELDS = ("UB", "VB", "TempB", "QMixB", "PsB", "UN", "VN", "TempN", "QMixN", "PsN")
+
+
+def _restart_shape(name, im, jm, km, nq):
+    if name.startswith("Ps"):
+        return (jm, im)
+    if name.startswith("QMix"):
+        return (nq, km, jm, im)
+    return (km, jm, im)
+
+
+def restart_write(path, time, lon, lat, sig, fields, nq: int = 1):
+    """Writes the restart state (RestartFileOutput): fields maps each of
+    RESTART_FIELDS to a float64 array, C order (lon fastest)."""
+    lon, lat, sig = (np.ascontiguousarray(x, dtype=np.float64) for x in (lon, lat, sig))
+    im, jm, km = lon.size, lat.size, sig.size
+    arrs = []
+    for nm in RESTART_FIELDS:
+        a = np.ascontiguousarray(fields[nm], dtype=np.float64)
+        if a.shape != _restart_shape(nm, im, jm, km, nq):
+            raise ValueError(f"{nm}: shape {a.shape}, expected {_restart_shape(nm, im, jm, km, nq)}")
+        arrs.append(a)
+    d = _RestartDesc(im, jm, km, nq, time, lon.ctypes.data, lat.ctypes.data, sig.ctypes.data)
+    ptrs = (_vp * 10)(*[a.ctypes.data for a in arrs])
+    check(lib().sht_restart_write(str(path).encode(), ctypes.byref(d), ptrs))
+
+
+def restart_read(path, im: int, jm: int, km: int, nq: int = 1):
+    """Reads the restart state (RestartFileGet): (time, {name: array})."""
+    out = {nm: np.empty(_restart_shape(nm, im, jm, km, nq)) for nm in RESTART_FIELDS}
+    d = _RestartDesc(im, jm, km, nq, 0.0, None, None, None)
+    ptrs = (_vp * 10)(*[out[nm].ctypes.data for nm in RESTART_FIELDS])
+    t = ctypes.c_double()
+    check(lib().sht_restart_read(str(path).encode(), ctypes.byref(d), ctypes.byref(t), ptrs))
+    return t.value, out
 
 
 class Batch:

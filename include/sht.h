@@ -181,6 +181,27 @@ int sht_lat_cell_edges(const double *lat_weight, int jm, double *edges);
 int sht_exchange_grid(const double *lat_a, const double *w_a, int jm_a, const double *lat_b,
                       const double *w_b, int jm_b, double *lat, double *lat_weight, int *jm_out);
 
+/* ---- restart state: RestartFileOutput / RestartFileGet of the dynamical
+ *      core (atm/dcpam_main_mod.f90:3063-3066 and :4134-4137), the grid
+ *      fields of the two leapfrog time levels B and N, in a netCDF classic
+ *      file (64-bit offset form) written and read without a netCDF library.
+ *      Host arrays; lon fastest as in Fortran: xyz(0:im-1, 1:jm, 1:km) for
+ *      U, V, Temp; xyzf(0:im-1, 1:jm, 1:km, 1:nq) for QMix; xy(0:im-1, 1:jm)
+ *      for Ps.  Variables UB, VB, TempB, QMixB, PsB, UN, VN, TempN, QMixN,
+ *      PsN over dimensions (nq, sig, lat, lon), with coordinate variables lon,
+ *      lat, sig and a global attribute time.  (gtool5's own names are not in
+ *      the reference tree.)                                                 */
+typedef struct {
+  int im, jm, km, nq;           /* longitudes, latitudes, levels, tracers   */
+  double time;                  /* restart time in seconds (write)          */
+  const double *lon, *lat, *sig;/* im, jm, km coordinates (write only)      */
+} sht_restart_desc;
+/* fields[10] in the order UB, VB, TempB, QMixB, PsB, UN, VN, TempN, QMixN, PsN */
+int sht_restart_write(const char *path, const sht_restart_desc *desc, const double *const fields[10]);
+/* Reads the fields of a file whose grid matches desc's im, jm, km, nq (any
+ * fields[i] may be NULL to skip it) and the restart time (time may be NULL). */
+int sht_restart_read(const char *path, const sht_restart_desc *desc, double *time, double *const fields[10]);
+
 /* ---- ensemble batches: many members of uneven truncation and level count
  *      transformed by one hipGraph launch.  There is no SPML counterpart (SPML
  *      holds one global plan); this is the batched form of wa_xya/xya_wa for

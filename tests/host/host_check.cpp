@@ -16,6 +16,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 #include "sht_host.h"
 
 using namespace sht::host;
@@ -227,12 +229,68 @@ static void admission() {
   CHECK(huge.submit(0, 1, 0, nullptr, nullptr, 1, 0.0, &t0) == SHT_ERR_BUSY);  // no wrap-around
 }
 
+static void restart() {
+  const RestartDims d{8, 4, 3, 2};
+  const size_t nxy = 32, nxyz = 96;
+  std::vector<std::vector<double>> f(10);
+  std::vector<const double *> in(10);
+  std::vector<double *> out(10);
+  std::vector<std::vector<double>> back(10);
+  std::mt19937 rng(3);
+  std::normal_distribution<double> nd;
+  for (int i = 0; i < 10; ++i) {
+    const size_t n = (i % 5 == 4) ? nxy : (i % 5 == 3) ? nxyz * 2 : nxyz;
+    f[i].resize(n);
+    for (double &x : f[i]) x = nd(rng);
+    back[i].assign(n, 0.0);
+    in[i] = f[i].data();
+    out[i] = back[i].data();
+  }
+  const double lon[8] = {0, 1, 2, 3, 4, 5, 6, 7}, lat[4] = {-1, -0.3, 0.3, 1}, sig[3] = {0.9, 0.5, 0.1};
+  char path[] = "/tmp/sht_rst_XXXXXX";
+  const int fd = mkstemp(path);
+  CHECK(fd >= 0);
+  close(fd);
+  std::string msg;
+  CHECK(restart_write(path, d, 1234.5, lon, lat, sig, in.data(), msg) == SHT_OK);
+  double t = 0.0;
+  CHECK(restart_read(path, d, &t, out.data(), msg) == SHT_OK && t == 1234.5);
+  for (int i = 0; i < 10; ++i) CHECK(back[i] == f[i]);
+  RestartDims other = d;
+  other.km = 4;
+  CHECK(restart_read(path, other, &t, out.data(), msg) == SHT_ERR_INVALID && !msg.empty());
+  // every truncation of the file is refused without reading past its end
+  FILE *fp = std::fopen(path, "rb");
+  std::vector<char> bytes;
+  int c;
+  while ((c = std::fgetc(fp)) != EOF) bytes.push_back((char)c);
+  std::fclose(fp);
+  for (size_t cut = 0; cut < bytes.size(); cut += 1 + cut / 8) {
+    FILE *g = std::fopen(path, "wb");
+    std::fwrite(bytes.data(), 1, cut, g);
+    std::fclose(g);
+    CHECK(restart_read(path, d, &t, out.data(), msg) != SHT_OK);
+  }
+  // random corruption of the header never reads out of bounds
+  for (int trial = 0; trial < 300; ++trial) {
+    std::vector<char> x = bytes;
+    for (int k = 0; k < 4; ++k) x[rng() % 600] = (char)rng();
+    FILE *g = std::fopen(path, "wb");
+    std::fwrite(x.data(), 1, x.size(), g);
+    std::fclose(g);
+    (void)restart_read(path, d, &t, out.data(), msg);
+  }
+  std::remove(path);
+  CHECK(restart_write("/nonexistent-dir/x.nc", d, 0.0, lon, lat, sig, in.data(), msg) == SHT_ERR_INVALID);
+}
+
 int main() {
   descriptors();
   indices();
   tune_cache();
   exchange();
   admission();
+  restart();
   setenv("SHT_TEST_COUNT", "3", 1);
   CHECK(injected_count("SHT_TEST_COUNT") == 3);
   setenv("SHT_TEST_COUNT", "3x", 1);

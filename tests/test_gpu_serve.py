@@ -104,9 +104,10 @@ def test_plan_creation_failure_cleans_up(monkeypatch, k):
         with pytest.raises(shtamd.ShtError) as e:
             shtamd.Plan(42, 128, 64, 8, precision=prec, flags=shtamd.FLAG_GRAD)
         assert e.value.code == shtamd.SHT_ERR_NOMEM
-    with pytest.raises(shtamd.ShtError) as e:
-        shtamd.Server([(21, 64, 32), (42, 128, 64)], slots=2, max_fields=8)
-    assert e.value.code == shtamd.SHT_ERR_NOMEM
+    if k <= 13:  # a server's FP64 plans (no derivative tables) allocate 13 times
+        with pytest.raises(shtamd.ShtError) as e:
+            shtamd.Server([(21, 64, 32), (42, 128, 64)], slots=2, max_fields=8)
+        assert e.value.code == shtamd.SHT_ERR_NOMEM
     monkeypatch.delenv("SHT_INJECT_PLAN_FAILURE")
     p = shtamd.Plan(42, 128, 64, 8, flags=shtamd.FLAG_GRAD)
     spec = serve.synth.spectra(42, 8, cfg=9)

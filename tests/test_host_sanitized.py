@@ -1,6 +1,7 @@
 """The host-only logic of libsht_hip.so (descriptor validation, SPML index
 helpers, the shape-cache parser, gmapgen's exchange grid, the request
-server's admission and batching queue) built alone with AddressSanitizer and
+server's admission and batching queue, the restart file writer and reader)
+built alone with AddressSanitizer and
 UndefinedBehaviorSanitizer and driven by tests/host/host_check.cpp on the
 CPU, several threads included.  No GPU: these sources use no HIP."""
 import os
@@ -19,7 +20,8 @@ def test_host_logic_under_asan_ubsan(tmp_path):
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fno-omit-frame-pointer",
            "-fsanitize=address,undefined", "-fno-sanitize-recover=all", "-pthread",
            f"-I{CSRC}", f"-I{os.path.join(ROOT, 'include')}",
-           os.path.join(ROOT, "tests", "host", "host_check.cpp"), os.path.join(CSRC, "sht_host.cpp"), "-o", exe]
+           os.path.join(ROOT, "tests", "host", "host_check.cpp"), os.path.join(CSRC, "sht_host.cpp"),
+           os.path.join(CSRC, "sht_restart.cpp"), "-o", exe]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1", UBSAN_OPTIONS="print_stacktrace=1")
