@@ -159,13 +159,20 @@ def test_autotune_and_cache(tmp_path):
     assert inv in SHAPES and fwd in SHAPES
     assert ti > 0 and tf > 0
     data = json.load(open(cache))
-    assert data[f"T{N}_{im}x{jm}_nf{nf}_p0"] == [inv, fwd]
+    entry = data[f"T{N}_{im}x{jm}_nf{nf}_p0"]
+    assert entry[:2] == [inv, fwd] and len(entry) == 6
+    # the wavenumber classes of the split shape are kept with the shapes
+    for d, (m4, m2) in ((shtamd.DIR_INVERSE, entry[2:4]), (shtamd.DIR_FORWARD, entry[4:6])):
+        if m2 >= 0:
+            assert p.get_classes(nf, d) == (m4, m2) and 0 <= m4 <= m2 <= N + 1
     # other field counts keep the default; an entry for nf only
     assert p.get_shapes(nf) == (inv, fwd)
     # a second plan takes the cached choice without timing
     q = shtamd.Plan(N, im, jm, nf)
     (inv2, fwd2), (t2i, t2f) = q.autotune(nf, reps=10, cache_path=cache)
     assert (inv2, fwd2) == (inv, fwd) and t2i == 0.0 and t2f == 0.0
+    for d in (shtamd.DIR_INVERSE, shtamd.DIR_FORWARD):
+        assert q.get_classes(nf, d) == p.get_classes(nf, d)
     # and the tuned plan still meets the FP64 bar
     spec = synth.spectra(N, nf, cfg=7)
     ref = oracle.Plan(N, im, jm)
@@ -261,4 +268,31 @@ def test_size_dependent_paths_bitwise_equal(prec):
         s_big = run_forward(p, g_big, big)
         s_small = run_forward(p, g_big[:small], small)
         assert np.array_equal(s_big[:, :small], s_small)
+    p.close()
+
+
+@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (42, 128, 64, 40)])
+def test_split_classes_meet_the_bar(N, im, jm, nf):
+    """Every wavenumber class split the autotuner may choose (and a few
+    extremes: all four-way, all two-way, none) keeps the split shape within
+    the FP64 bar in both directions."""
+    spec = synth.spectra(N, nf, cfg=17)
+    ref = oracle.Plan(N, im, jm)
+    g_ref = ref.spec_to_grid(spec)
+    s_ref = ref.grid_to_spec(g_ref)
+    p = shtamd.Plan(N, im, jm, nf)
+    p.set_shapes(nf, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_TILE_SPLIT)
+    M = N + 1
+    for m4, m2 in [(0, M // 5), (M // 10, M // 2), (M, M), (0, M), (0, 0), (M // 3, M // 3)]:
+        say("classes", N, nf, m4, m2)
+        for d in (shtamd.DIR_INVERSE, shtamd.DIR_FORWARD):
+            p.set_classes(nf, d, m4, m2)
+            assert p.get_classes(nf, d) == (m4, m2)
+        assert rel(run_inverse(p, spec, nf)[0], g_ref) < TOL
+        assert rel(run_forward(p, g_ref, nf), s_ref) < TOL
+    with pytest.raises(shtamd.ShtError):
+        p.set_classes(nf, shtamd.DIR_INVERSE, 5, 2)
+    with pytest.raises(shtamd.ShtError):
+        p.set_classes(nf, shtamd.DIR_FORWARD, 0, M + 1)
+    p.set_classes(nf, shtamd.DIR_INVERSE, -1, -1)  # back to the built-in classes
     p.close()
