@@ -286,10 +286,13 @@ def legendre_kernel_name(stage: str, fp64: bool, shape: int) -> str:
 ENSEMBLE_GRIDS = {42: (128, 64), 85: (256, 128), 170: (512, 256)}
 
 
-def run_ensemble(precision: str, nmember: int, steps: int, warmup: int, seed: int = 20261202):
+def run_ensemble(precision: str, nmember: int, steps: int, warmup: int, seed: int = 20261202,
+                 shapes_only: bool = False):
     """Ensemble of nmember members, truncation uniform in {T42, T85, T170},
     levels uniform in {16, 32, 48} (3L+1 fields each), one graph launch per
-    round trip; returns a JSON-ready dict."""
+    round trip; returns a JSON-ready dict.  shapes_only: the autotuner picks
+    one Legendre shape per direction and group, with the built-in wavenumber
+    classes; otherwise FP64 groups also tune the split shape's classes."""
     import torch
 
     import oracle
@@ -308,7 +311,7 @@ def run_ensemble(precision: str, nmember: int, steps: int, warmup: int, seed: in
         hipmem.put_member_spectra(spec_in, ld, s)
         specs.append(s)
     # per-group Legendre shapes from the autotuner, then the graph is captured
-    b.autotune(reps=5)
+    b.autotune(reps=5, shapes_only=shapes_only)
     stream = torch.cuda.current_stream()
     for _ in range(max(1, warmup)):
         b.run(shtamd.DIR_ROUNDTRIP, stream)
@@ -340,6 +343,7 @@ def run_ensemble(precision: str, nmember: int, steps: int, warmup: int, seed: in
     return {
         "metric": "ensemble SHT fields/s (64 members, T42/T85/T170 x L16/32/48, one graph launch)",
         "value": round(nf_total / (best * 1e-3), 1), "unit": "fields/s", "precision": precision,
+        "tuning": "shapes per direction" if shapes_only else "shapes and wavenumber classes",
         "members": nmember, "fields": nf_total,
         "members_per_truncation": {f"T{t}": int((Ts == t).sum()) for t in (42, 85, 170)},
         "ms_per_round_trip": {k: round(v, 4) for k, v in times.items()},
@@ -378,8 +382,14 @@ def main():
     import oracle
 
     if args.workload == "ensemble":
-        prec = "bf16x6" if args.precision == "auto" else args.precision
-        print(json.dumps(run_ensemble(prec, args.members, max(1, args.steps // 10), args.warmup)))
+        # the headline form (split-BF16 x6), then FP64 tuned two ways: one
+        # shape per direction, and shapes with the split shape's wavenumber
+        # classes (FP64 only); one JSON line each
+        precs = ["bf16x6", "fp64"] if args.precision == "auto" else [args.precision]
+        for prec in precs:
+            for shapes_only in ((False,) if prec != "fp64" else (True, False)):
+                print(json.dumps(run_ensemble(prec, args.members, max(1, args.steps // 10), args.warmup,
+                                              shapes_only=shapes_only)), flush=True)
         return
     if args.workload == "serve":
         prec = "bf16x6" if args.precision == "auto" else args.precision

@@ -214,12 +214,16 @@ int sht_batch_run(sht_batch *b, int dir, void *stream) {
 }
 
 int sht_batch_autotune(sht_batch *b, int reps, const char *cache_path) {
+  return sht_batch_autotune_flags(b, reps, cache_path, 0);
+}
+
+int sht_batch_autotune_flags(sht_batch *b, int reps, const char *cache_path, int flags) {
   if (!b) return fail(SHT_ERR_INVALID, "sht_batch_autotune: batch is NULL");
   // the tuner runs on a stream of its own through each group plan's Fourier
   // scratch: a graph or eager run still in flight on any stream would share it
   SHT_HIP(hipDeviceSynchronize());
   for (auto &g : b->groups) {
-    int rc = sht_plan_autotune(g.plan, g.nf, reps, cache_path, nullptr);
+    int rc = sht_plan_autotune_flags(g.plan, g.nf, reps, cache_path, flags, nullptr);
     if (rc) return rc;
   }
   // graphs captured before hold the old kernels: capture again on the next run

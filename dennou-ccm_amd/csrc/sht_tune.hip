@@ -94,10 +94,10 @@ struct Candidate {
 // the wavenumbers (heaviest first), with no four-way class or one over its
 // first quarter or half.  The work per m falls with m, so the boundaries are
 // counts of the heaviest wavenumbers.
-std::vector<Candidate> candidates(const Plan &p, int dir) {
+std::vector<Candidate> candidates(const Plan &p, int dir, bool classes) {
   std::vector<Candidate> c;
   for (int s = 0; s < nshapes(p); ++s) c.push_back({s, KSplit{-1, -1}});
-  if (p.prec != SHT_PREC_FP64 || p.N < 8) return c;
+  if (!classes || p.prec != SHT_PREC_FP64 || p.N < 8) return c;
   const KSplit def = default_classes(p, dir);
   for (int pct = 20; pct <= 70; pct += 10) {
     const int m2 = std::max(1, (p.N + 1) * pct / 100);
@@ -174,12 +174,20 @@ int sht_plan_get_classes(const sht_plan *h, int nf, int dir, int *m4, int *m2) {
 }
 
 int sht_plan_autotune(sht_plan *h, int nf, int reps, const char *cache_path, double *best_us) {
+  return sht_plan_autotune_flags(h, nf, reps, cache_path, 0, best_us);
+}
+
+int sht_plan_autotune_flags(sht_plan *h, int nf, int reps, const char *cache_path, int flags,
+                            double *best_us) {
   if (!h) return fail(SHT_ERR_INVALID, "sht_plan_autotune: plan is NULL");
   Plan &p = h->p;
   if (nf < 1 || nf > p.kmax) return fail(SHT_ERR_INVALID, "sht_plan_autotune: nf out of range");
   if (reps < 1 || reps > 10000) return fail(SHT_ERR_INVALID, "sht_plan_autotune: reps out of range");
+  if (flags & ~SHT_TUNE_SHAPES_ONLY) return fail(SHT_ERR_INVALID, "sht_plan_autotune: unknown flags");
+  const bool classes = !(flags & SHT_TUNE_SHAPES_ONLY);
   if (best_us) best_us[0] = best_us[1] = 0.0;
-  const std::string key = cache_key(p, nf);
+  // shapes-only choices are cached under a key of their own
+  const std::string key = cache_key(p, nf) + (classes ? "" : "_shapes");
   std::map<std::string, host::Tuning> cache;
   if (cache_path) {
     cache = read_cache(cache_path);
@@ -195,7 +203,7 @@ int sht_plan_autotune(sht_plan *h, int nf, int reps, const char *cache_path, dou
   }
   Candidate best[2] = {{0, KSplit{-1, -1}}, {0, KSplit{-1, -1}}};
   double tbest[2] = {0.0, 0.0};
-  const std::vector<Candidate> cand[2] = {candidates(p, 0), candidates(p, 1)};
+  const std::vector<Candidate> cand[2] = {candidates(p, 0, classes), candidates(p, 1, classes)};
   if (cand[0].size() > 1 || cand[1].size() > 1) {
     int cur = 0;
     SHT_HIP(hipGetDevice(&cur));

@@ -30,6 +30,7 @@ SHAPE_AUTO, SHAPE_KSPLIT, SHAPE_WAVE, SHAPE_TILE, SHAPE_TILE_SPLIT, SHAPE_STAGED
 SHAPES = ("ksplit", "wave", "tile", "tile_split", "staged")
 NSHAPE = len(SHAPES)
 FLAG_GRAD = 1
+TUNE_SHAPES_ONLY = 1
 
 _lib = None
 
@@ -118,6 +119,8 @@ SIGNATURES = {
     "sht_plan_set_shapes": (_ci, [_vp, _ci, _ci, _ci]),
     "sht_plan_get_shapes": (_ci, [_vp, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_plan_autotune": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _dp]),
+    "sht_plan_autotune_flags": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _ci, _dp]),
+    "sht_batch_autotune_flags": (_ci, [_vp, _ci, ctypes.c_char_p, _ci]),
     "sht_plan_set_classes": (_ci, [_vp, _ci, _ci, _ci, _ci]),
     "sht_plan_get_classes": (_ci, [_vp, _ci, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_zonal_create": (_ci, [_ci, _ci, _ci, ctypes.POINTER(_vp)]),
@@ -302,12 +305,12 @@ class Plan:
         check(lib().sht_plan_get_classes(self._h, nfield, direction, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
-    def autotune(self, nfield: int, reps: int = 20, cache_path: str | None = None):
+    def autotune(self, nfield: int, reps: int = 20, cache_path: str | None = None, shapes_only: bool = False):
         """Time every shape at nfield fields and keep the fastest.  Returns
         ((inv_shape, fwd_shape), (inv_us, fwd_us)); the times are 0 on a cache hit."""
         us = (ctypes.c_double * 2)()
         path = cache_path.encode() if cache_path else None
-        check(lib().sht_plan_autotune(self._h, nfield, reps, path, us))
+        check(lib().sht_plan_autotune_flags(self._h, nfield, reps, path, TUNE_SHAPES_ONLY if shapes_only else 0, us))
         return self.get_shapes(nfield), (us[0], us[1])
 
     # ---- host drop-in (SPML packing), numpy arrays
@@ -496,9 +499,12 @@ class Batch:
     def run(self, direction: int = DIR_ROUNDTRIP, stream=None):
         check(lib().sht_batch_run(self._h, direction, _stream_handle(stream)))
 
-    def autotune(self, reps: int = 10, cache_path: str | None = None):
-        """Tune every group's Legendre shapes at its field count; recapture on the next run."""
-        check(lib().sht_batch_autotune(self._h, reps, cache_path.encode() if cache_path else None))
+    def autotune(self, reps: int = 10, cache_path: str | None = None, shapes_only: bool = False):
+        """Tune every group's Legendre shapes (and, for FP64 groups unless
+        shapes_only, the split shape's wavenumber classes) at its field count;
+        recapture on the next run."""
+        check(lib().sht_batch_autotune_flags(self._h, reps, cache_path.encode() if cache_path else None,
+                                             TUNE_SHAPES_ONLY if shapes_only else 0))
 
     def run_eager(self, direction: int = DIR_ROUNDTRIP, stream=None):
         check(lib().sht_batch_run_eager(self._h, direction, _stream_handle(stream)))

@@ -237,6 +237,8 @@ int sht_batch_run_eager(sht_batch *batch, int direction, void *stream);
  * captured graphs so the next sht_batch_run captures the tuned kernels.
  * Synchronizes the device; call it while the batch is idle. */
 int sht_batch_autotune(sht_batch *batch, int reps, const char *cache_path);
+/* The same with sht_plan_autotune_flags' flags. */
+int sht_batch_autotune_flags(sht_batch *batch, int reps, const char *cache_path, int flags);
 
 /* ---- request server: continuous batching of transform requests whose data
  *      are resident in HBM.  No SPML counterpart (SPML is a library called by
@@ -366,6 +368,12 @@ int sht_plan_get_classes(const sht_plan *plan, int nf, int dir, int *m4, int *m2
  * direction times in microseconds (0 when taken from the cache).  It uses the plan's
  * scratch: call it while no other call on the plan is in flight. */
 int sht_plan_autotune(sht_plan *plan, int nf, int reps, const char *cache_path, double *best_us);
+/* The same with flags: SHT_TUNE_SHAPES_ONLY times the shapes with their
+ * built-in wavenumber classes only (one choice per direction, cached under
+ * the key with the suffix "_shapes"); 0 = sht_plan_autotune. */
+enum { SHT_TUNE_SHAPES_ONLY = 1 };
+int sht_plan_autotune_flags(sht_plan *plan, int nf, int reps, const char *cache_path, int flags,
+                            double *best_us);
 
 #ifdef __cplusplus
 }
