@@ -26,7 +26,8 @@ FLAGS = [
 ]
 SOURCES = ["sht_capi.hip", "sht_tables.hip", "sht_fft.hip", "sht_legendre.hip",
            "sht_legendre_mixed.hip", "sht_pack.hip", "sht_batch.hip", "sht_server.hip",
-           "sht_tune.hip", "sht_remap.hip", "sht_zonal.hip", "sht_exchange.hip"]
+           "sht_tune.hip", "sht_remap.hip", "sht_zonal.hip", "sht_exchange.hip",
+           "sht_noalloc.hip"]
 
 
 # host-only C++ (no HIP), compiled by the host compiler

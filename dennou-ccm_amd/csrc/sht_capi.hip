@@ -117,6 +117,17 @@ static int grid_to_spec(Plan &p, const double *grid, double *spec, int nf, hipSt
   });
 }
 
+// spectral -> grid with the Fourier intermediate at `four` (the static
+// scratch of sht_spec_to_grid_dev_noalloc) instead of the plan's; the same
+// kernels as spec_to_grid, without stage timers (they create events).
+int spec_to_grid_into(Plan &p, const double *spec, double *grid, int nf, void *four, hipStream_t s) {
+  int rc = p.prec == SHT_PREC_FP64
+               ? launch_leg_inv_f64(p, spec, static_cast<double *>(four), nullptr, true, false, nf, s)
+               : launch_leg_inv_mixed(p, spec, static_cast<float *>(four), nf, s);
+  if (rc) return rc;
+  return launch_fft_inv(p, four, nullptr, p.prec != SHT_PREC_FP64, grid, nullptr, nullptr, nf, s);
+}
+
 // Gradients always run the FP64 Legendre step (the derivative table is FP64 only).
 static int spec_to_grad(Plan &p, const double *spec, double *grid, double *glon, double *glat,
                         int nf, hipStream_t s) {
@@ -194,6 +205,9 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
     delete h;
     return fail(SHT_ERR_DEVICE, std::string("hipGetDevice: ") + hipGetErrorString(e));
   }
+  // the static scratch of sht_spec_to_grid_dev_noalloc on this device; a
+  // failed lookup leaves that entry point reporting it, not plan creation
+  (void)noalloc_init_device(p.device);
   rc = build_tables(p);
   if (!rc) p.fft_fwd_resident = fft_fwd_capacity(p);
   if (!rc && p.prec != SHT_PREC_FP64) rc = build_mixed_tables(p);

@@ -246,6 +246,11 @@ int launch_leg_fwd_mixed_shape(const Plan &p, int shape, const float *four, doub
                                hipStream_t s);
 int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s);
 
+// Allocation-free spectral -> grid (sht_noalloc.hip): the static scratch's
+// device address is looked up once per device at plan creation.
+int noalloc_init_device(int device);
+int spec_to_grid_into(Plan &p, const double *spec, double *grid, int nf, void *four, hipStream_t s);
+
 // Layout helpers (sht_pack.hip)
 int launch_spml_to_canon(const Plan &p, const double *wa, double *spec, int nf, hipStream_t s);
 int launch_canon_to_spml(const Plan &p, const double *spec, double *wa, int nf, hipStream_t s);

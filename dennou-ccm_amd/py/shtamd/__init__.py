@@ -82,6 +82,7 @@ SIGNATURES = {
     "sht_nm_l": (_ci, [_ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
     "sht_spec_to_grid_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_grid_to_spec_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
+    "sht_spec_to_grid_dev_noalloc": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_spec_to_grad_dev": (_ci, [_vp, _vp, _vp, _vp, _vp, _ci, _vp]),
     "sht_grid_to_div_dev": (_ci, [_vp, _vp, _vp, _vp, _ci, _vp]),
     "sht_spec_lapla_dev": (_ci, [_vp, _vp, _vp, _ci, _ci, _vp]),
@@ -240,6 +241,12 @@ class Plan:
     def spec_to_grid(self, spec, grid, nfield: int, stream=None):
         check(lib().sht_spec_to_grid_dev(self._h, _dev(spec, self._ns(nfield), "spec"),
                                          _dev(grid, self._ng(nfield), "grid"), nfield, _stream_handle(stream)))
+
+    def spec_to_grid_noalloc(self, spec, grid, nfield: int, stream=None):
+        """spec_to_grid through the library's static scratch (allocates nothing)."""
+        check(lib().sht_spec_to_grid_dev_noalloc(self._h, _dev(spec, self._ns(nfield), "spec"),
+                                                 _dev(grid, self._ng(nfield), "grid"), nfield,
+                                                 _stream_handle(stream)))
 
     def grid_to_spec(self, grid, spec, nfield: int, stream=None):
         check(lib().sht_grid_to_spec_dev(self._h, _dev(grid, self._ng(nfield), "grid"),

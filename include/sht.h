@@ -106,6 +106,18 @@ int sht_nm_l(int l, int *n, int *m);
 /* xya_wa: spectral -> grid */
 int sht_spec_to_grid_dev(sht_plan *plan, const double *spec, double *grid,
                          int nfield, void *stream);
+/* xya_wa like sht_spec_to_grid_dev, allocating nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory at call time
+ * (HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The Fourier intermediate goes to a static 32 MiB device array per GPU
+ * (looked up at the first sht_plan_create on that device) instead of the
+ * plan's scratch; calls whose intermediate (jm/2 * ceil(nfield/G) * (nmax+1)
+ * * 4G values, G = 2048/im, 8 bytes each in FP64 plans, 4 in the mixed
+ * forms) exceeds it fail with SHT_ERR_INVALID.  All calls on one GPU share the
+ * array: calls that overlap on one GPU race.  Results equal
+ * sht_spec_to_grid_dev's bitwise. */
+int sht_spec_to_grid_dev_noalloc(sht_plan *plan, const double *spec, double *grid,
+                                 int nfield, void *stream);
 /* wa_xya: grid -> spectral */
 int sht_grid_to_spec_dev(sht_plan *plan, const double *grid, double *spec,
                          int nfield, void *stream);
