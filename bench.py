@@ -269,6 +269,27 @@ def traffic_from_profile(form: str, kernel_prefix: str):
     return None
 
 
+def mfma_from_profile(form: str):
+    """Measured matrix-core utilisation of the Legendre kernels from the
+    committed rocprofv3 counter summary profiles/t85l32_<form>_mfma.md
+    (SQ_VALU_MFMA_BUSY_CYCLES over the SIMD-cycles of the profiled duration),
+    {stage: {"kernel": name, "util": fraction}}, or None when it is absent."""
+    path = os.path.join(ROOT, "profiles", f"t85l32_{form}_mfma.md")
+    if not os.path.exists(path):
+        return None
+    want = {"leg_inv": "leg_inv_" + ("f64" if form == "fp64" else "mixed"),
+            "leg_fwd": "leg_fwd_" + ("f64" if form == "fp64" else "mixed")}
+    out = {}
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) < 6 or not cells[5].endswith("%"):
+            continue
+        for stage, prefix in want.items():
+            if cells[0].startswith(prefix) and stage not in out:
+                out[stage] = {"kernel": cells[0], "util": round(float(cells[5].rstrip(" %")) / 100, 4)}
+    return out or None
+
+
 KERNEL_NAMES = {"fft_inv": "fft_inv_kernel", "fft_fwd": "fft_fwd_kernel"}
 SHAPE_NAMES = ("ksplit", "wave", "tile", "tile_split", "staged")
 
@@ -450,6 +471,8 @@ def main():
             if k.startswith("leg"):
                 peak = FP64_PEAK_TFLOPS if form == "fp64" else BF16_PEAK_TFLOPS
                 t.setdefault("legendre_mfma_frac", {})[k] = round(f / (ms * 1e-3) / 1e12 / peak, 4)
+        # measured by rocprofv3 counters (matrix-core busy cycles), built-in shapes
+        t["legendre_mfma_util_measured"] = mfma_from_profile(form)
         per_form[form] = t
     fp64_v = per_form["fp64"]["fields_per_s"]
 
