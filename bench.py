@@ -71,7 +71,7 @@ def rel(a, b):
     return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
 
 
-def run_gpu(precision: int, steps: int, warmup: int, autotune: bool = True):
+def run_gpu(precision: int, steps: int, warmup: int, autotune: bool = True, fused: bool = False):
     import torch
 
     import shtamd
@@ -83,6 +83,8 @@ def run_gpu(precision: int, steps: int, warmup: int, autotune: bool = True):
     if autotune:
         shapes, _ = plan.autotune(NF, reps=20)  # Legendre kernel shapes for this batch
     else:
+        if fused and precision != 0:  # the fused inverse of the mixed forms
+            plan.set_shapes(0, shtamd.SHAPE_FUSED, shtamd.SHAPE_AUTO)
         shapes = plan.get_shapes(NF)  # the built-in choice
     s_in = torch.from_numpy(shtamd.spec_to_real(spec_c)).to(dev)
     grid = torch.empty((NF, JM, IM), dtype=torch.float64, device=dev)
@@ -291,12 +293,14 @@ def mfma_from_profile(form: str):
 
 
 KERNEL_NAMES = {"fft_inv": "fft_inv_kernel", "fft_fwd": "fft_fwd_kernel"}
-SHAPE_NAMES = ("ksplit", "wave", "tile", "tile_split", "staged")
+SHAPE_NAMES = ("ksplit", "wave", "tile", "tile_split", "staged", "fused")
 
 
 def legendre_kernel_name(stage: str, fp64: bool, shape: int) -> str:
     """Kernel name (as rocprof prints it, up to the template list) of a
     Legendre stage in a given shape."""
+    if not fp64 and stage == "leg_inv" and shape == 5:
+        return "inv_fused_kernel"  # Legendre step and FFT in one launch
     if fp64:
         suffix = ("_f64_kernel", "_f64_wave_kernel", "_f64_tile_kernel", "_f64_tile_kernel",
                   "_f64_staged_kernel")[shape]
@@ -387,9 +391,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests in the 500 req/s serving run of the headline line (0 = skip)")
     ap.add_argument("--members", type=int, default=64)
-    ap.add_argument("--shapes", default="auto", choices=["auto", "default"],
+    ap.add_argument("--shapes", default="auto", choices=["auto", "default", "fused"],
                     help="Legendre kernel shapes: autotuned per form (auto) or the built-in choice "
-                         "(default; what the rocprof summaries use, without the tuner's trial launches)")
+                         "(default; what the rocprof summaries use, without the tuner's trial launches); "
+                         "fused: the built-in choice with the fused inverse for the mixed forms")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
@@ -424,7 +429,8 @@ def main():
     ref = oracle.Plan(N, IM, JM)
     results = {}
     for form in forms:
-        res = run_gpu(PRECISIONS[form], args.steps, args.warmup, autotune=args.shapes == "auto")
+        res = run_gpu(PRECISIONS[form], args.steps, args.warmup, autotune=args.shapes == "auto",
+                      fused=args.shapes == "fused")
         g_ref = ref.spec_to_grid(res["spec_in"])
         res["err"] = {"grid_rel_l2": rel(res["grid"], g_ref),
                       "spec_rel_l2": rel(res["spec"], ref.grid_to_spec(res["grid"])),
@@ -439,11 +445,17 @@ def main():
     # roofline of the dominant kernel of the headline form
     tb = table_bytes(prec)
     alg = algorithmic(tb, NF, 16 if prec == 0 else 8)
+    hshapes = results[headline]["shapes"]
+    if prec != 0 and hshapes[0] == 5:
+        # the fused inverse: one launch, timed as the Legendre stage, doing the
+        # work of both inverse stages (the intermediate is still written and
+        # read once), so its algorithmic bytes are their sum
+        (lb, lf), (fb, ff) = alg["leg_inv"], alg["fft_inv"]
+        alg["leg_inv"] = (lb + fb, lf + ff)
     kms = {k: v for k, v in res["kernel_ms"].items() if v}
     dom = max(kms, key=kms.get)
     bytes_, flops = alg[dom]
     achieved = bytes_ / (kms[dom] * 1e-3) / 1e9
-    hshapes = results[headline]["shapes"]
     if dom in KERNEL_NAMES:
         # the forward FFT carries a third template argument (direct first pass)
         kprefix = f"{KERNEL_NAMES[dom]}<{IM},{'double' if prec == 0 else 'float'}"

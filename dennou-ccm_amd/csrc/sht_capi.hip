@@ -28,7 +28,7 @@ static void release(Plan &p) {
   void *ptrs[] = {p.mu, p.w, p.coslat, p.twid, p.Pinv, p.Hinv, p.Pfwd, p.poff_inv, p.poff_fwd,
                   p.Linv, p.Lfwd, p.loff_inv, p.loff_fwd, p.lexp_inv, p.lexp_fwd, p.four, p.four2,
                   p.stage_a, p.stage_b, p.fwd_units, p.zero, p.xcd_m[0], p.xcd_m[1],
-                  p.Hfwd};
+                  p.Hfwd, p.fsync};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p.pinned) (void)hipHostFree(p.pinned);
@@ -94,6 +94,11 @@ static int timed(Plan &p, int stage, hipStream_t s, F &&launch) {
 }
 
 static int spec_to_grid(Plan &p, const double *spec, double *grid, int nf, hipStream_t s) {
+  // the fused shape: one launch, timed as the Legendre stage
+  if (p.prec != SHT_PREC_FP64 && resolve_shape(p, nf, 0) == SHT_SHAPE_FUSED)
+    return timed(p, SHT_STAGE_LEG_INV, s, [&] {
+      return launch_inv_fused(p, spec, reinterpret_cast<float *>(p.four), grid, nf, s);
+    });
   int rc = timed(p, SHT_STAGE_LEG_INV, s, [&] {
     return p.prec == SHT_PREC_FP64
                ? launch_leg_inv_f64(p, spec, p.four, p.four2, true, false, nf, s)
@@ -121,6 +126,8 @@ static int grid_to_spec(Plan &p, const double *grid, double *spec, int nf, hipSt
 // scratch of sht_spec_to_grid_dev_noalloc) instead of the plan's; the same
 // kernels as spec_to_grid, without stage timers (they create events).
 int spec_to_grid_into(Plan &p, const double *spec, double *grid, int nf, void *four, hipStream_t s) {
+  if (p.prec != SHT_PREC_FP64 && resolve_shape(p, nf, 0) == SHT_SHAPE_FUSED)
+    return launch_inv_fused(p, spec, static_cast<float *>(four), grid, nf, s);
   int rc = p.prec == SHT_PREC_FP64
                ? launch_leg_inv_f64(p, spec, static_cast<double *>(four), nullptr, true, false, nf, s)
                : launch_leg_inv_mixed(p, spec, static_cast<float *>(four), nf, s);

@@ -80,6 +80,12 @@ struct Plan {
   // (work ~ K steps), one for the forward (work ~ 16-degree row tiles).
   int *xcd_m[2] = {nullptr, nullptr};
   int xcd_mlen[2] = {0, 0};
+  // Fused inverse (mixed forms): its queue counters and per-column-tile
+  // completion counts ((kFusedSyncHead + column tiles) slots of kSyncStride
+  // ints, zero between calls), and the workgroups the device holds at once
+  // (0: not available)
+  int *fsync = nullptr;
+  int fused_blocks = 0;
   // Legendre kernel shapes (SHT_SHAPE_*): the default for every field count,
   // and entries fixed by sht_plan_set_shapes / sht_plan_autotune for one count
   // and the wavenumber classes of the split shape (host::Tuning: K loop split
@@ -240,6 +246,12 @@ int launch_spec_lapla(const Plan &p, const double *in, double *out, int power, i
 // Legendre stage, mixed precision (sht_legendre_mixed.hip)
 int build_mixed_tables(Plan &p);
 int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s);
+// Fused inverse of the mixed forms (SHT_SHAPE_FUSED): Legendre step and FFT
+// in one persistent launch, grid values only; available for im in [64, 512]
+// and nmax < 192.
+constexpr int kFusedSyncHead = 18, kSyncStride = 64;
+bool fused_inv_available(const Plan &p);
+int launch_inv_fused(const Plan &p, const double *spec, float *four, double *grid, int nf, hipStream_t s);
 int launch_leg_inv_mixed_shape(const Plan &p, int shape, const double *spec, float *four, int nf,
                                hipStream_t s);
 int launch_leg_fwd_mixed_shape(const Plan &p, int shape, const float *four, double *spec, int nf,

@@ -26,8 +26,11 @@
 //   Linv per m: [par][piece][P2p pair rows][KP32 degrees]
 //   Lfwd per m: [par][piece][KR degree rows][P2p32 pairs]   (w/2 folded in)
 // The Fourier intermediate of mixed-precision plans is FP32.
+#include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
+#include "sht_fft_dev.h"
 #include "sht_internal.h"
 
 namespace sht {
@@ -208,11 +211,12 @@ constexpr int kRowPitch = 40;
 // group 0 again and are not stored).  Group g lands in LDS slot slot(g), so a
 // caller can stage only the groups that hold data.
 template <int MODE, int NIT, int NW, class Slot, class Load>
-__device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Slot slot, Load load) {
+__device__ __forceinline__ void stage_data(uint16_t *bt, int *ebits, int ngroups, Slot slot, Load load,
+                                           int tid) {
   constexpr int NP = Form<MODE>::NP;
-  const int lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
+  const int lane = tid & 63, c = lane & 15, k8 = lane >> 4;
   using T = decltype(load(0, 0, 0));  // double (spectra) or float (Fourier data)
-  for (int g0 = wave_id(); g0 < ngroups; g0 += NW * NIT) {
+  for (int g0 = __builtin_amdgcn_readfirstlane(tid >> 6); g0 < ngroups; g0 += NW * NIT) {
     T v[NIT][8];
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
@@ -289,20 +293,65 @@ constexpr int kOutPitch = 36;
 // workgroups) is resident at once (measured at T85: neutral at 97 fields, 6 %
 // faster at 388; the six-product form spills two registers for it).  With
 // more slices the hint would spill heavily.
-template <int MODE, int NSL, int CT, bool NT, int NW>
-__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu((CT == 1 && NSL <= 2) ? 5 : 1))) void
-leg_inv_mixed_kernel(
-    const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
-    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
-    int mlen, int N, int P2, int P2p, int ncol, int nct, FourGeom geo) {
+// Store of the FP32 Fourier intermediate: COH = device-coherent (agent-scope
+// relaxed atomic stores, 8 bytes each), for data another workgroup of the
+// same kernel reads; otherwise put_four.
+template <bool NT, bool COH>
+__device__ __forceinline__ void put_four4(float *dst, f4 v) {
+  if constexpr (COH) {
+    auto *u = reinterpret_cast<unsigned long long *>(dst);
+    const unsigned long long lo = (unsigned long long)__float_as_uint(v[1]) << 32 | __float_as_uint(v[0]);
+    const unsigned long long hi = (unsigned long long)__float_as_uint(v[3]) << 32 | __float_as_uint(v[2]);
+    __hip_atomic_store(u, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(u + 1, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    put_four<NT>(reinterpret_cast<f4 *>(dst), v);
+  }
+}
+template <bool NT, bool COH>
+__device__ __forceinline__ void put_four1(float *dst, float v) {
+  if constexpr (COH)
+    __hip_atomic_store(reinterpret_cast<unsigned *>(dst), __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  else
+    put_four<NT>(dst, v);
+}
+
+// LDS of one inverse workgroup item (bytes, 16-byte aligned parts): the split
+// data tiles, their FP16 exponents (scaled form only), the per-wave output
+// transpose.
+template <int MODE, int NSL, int CT, int NW>
+struct InvItemLds {
+  static constexpr int NP = Form<MODE>::NP, NG = 2 * NSL;
+  static constexpr size_t bt = sizeof(uint16_t) * CT * NG * NP * 16 * kRowPitch;
+  static constexpr size_t ebits = Form<MODE>::kScaled ? (sizeof(int) * CT * NG * 16 + 15) / 16 * 16 : 0;
+  static constexpr size_t obuf = sizeof(float) * NW * 16 * kOutPitch;
+  static constexpr size_t bytes = bt + ebits + obuf;
+};
+
+// Inverse: spec (FP64) -> four (FP32 even/odd Legendre sums).
+// Workgroup = one (m, CT 16-column tiles) of NW waves.  Each wave first issues
+// the table fragments of its first 16-pair row tile, then all threads split
+// the spectral tiles into LDS; wave w then takes row tiles w, w+NW, ... and runs
+// each against every column tile with the same table fragments.  CT = 2
+// halves the workgroups, and with them the waves that wait on memory at
+// once, and the table fragments fetched per column.  Results leave through a
+// per-wave LDS transpose as 16-byte stores of 4 consecutive columns (the
+// Fourier layout keeps a field group's 2G columns contiguous).  This is the
+// work of one (m, column tile t) item, with its LDS at lds; COH stores the
+// intermediate device-coherently (the fused kernel).
+template <int MODE, int NSL, int CT, bool NT, int NW, bool COH>
+__device__ __forceinline__ void leg_inv_mixed_item(
+    unsigned char *lds, const double *__restrict__ spec, float *__restrict__ four,
+    const uint16_t *__restrict__ L, const int *__restrict__ loff, const int *__restrict__ lexp, int N,
+    int P2, int P2p, int ncol, int nct, FourGeom geo, int m, int t, int tid) {
   constexpr int NP = Form<MODE>::NP, NG = 2 * NSL;  // LDS groups per column tile
-  __shared__ __align__(16) uint16_t bt[CT * NG * NP * 16 * kRowPitch];
-  __shared__ int ebits[CT * NG * 16];
-  __shared__ __align__(16) float obuf[NW * 16 * kOutPitch];  // per-wave output transpose
-  int m, t;
-  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
+  using Lds = InvItemLds<MODE, NSL, CT, NW>;
+  uint16_t *bt = reinterpret_cast<uint16_t *>(lds);
+  int *ebits = reinterpret_cast<int *>(lds + Lds::bt);
+  float *obuf = reinterpret_cast<float *>(lds + Lds::bt + Lds::ebits);
   const int ct0 = t * CT, nc = nct - ct0 < CT ? nct - ct0 : CT;
-  const int wave = wave_id(), lane = threadIdx.x & 63, r16 = lane & 15, kq = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63, r16 = lane & 15, kq = lane >> 4;
   const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
   const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
   const double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol + ct0 * 16;
@@ -338,7 +387,7 @@ leg_inv_mixed_kernel(
     const bool v = kk < (par ? ko : ke) && ct0 * 16 + cc < ncol;
     const double x = sm[(size_t)(v ? 2 * kk + par : 0) * ncol + (v ? cc : 0)];
     return v ? (float)x : 0.0f;  // split as FP32: see split(float)
-  });
+  }, tid);
   __syncthreads();
   for (int rt = wave; rt < rtiles; rt += NW) {
     if (rt != wave) load_a(rt);
@@ -375,7 +424,7 @@ leg_inv_mixed_kernel(
           const int q = lane + 64 * h, pl = q >> 3, par = (q >> 2) & 1, c4 = (q & 3) * 4;
           const int p = rt * 16 + pl, cc = (ct0 + j) * 16 + c4;
           const f4 v = *reinterpret_cast<const f4 *>(ob + pl * kOutPitch + par * 16 + c4);
-          if (p < P2 && cc < cend) put_four<NT>(reinterpret_cast<f4 *>(four + geo.at(p, m, par, cc)), v);
+          if (p < P2 && cc < cend) put_four4<NT, COH>(four + geo.at(p, m, par, cc), v);
         }
       } else if (col < ncol) {  // field groups of fewer than 4 columns (im = 2048)
 #pragma unroll
@@ -383,10 +432,213 @@ leg_inv_mixed_kernel(
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int p = rt * 16 + kq * 4 + i;
-            if (p < P2) put_four<NT>(four + geo.at(p, m, par, col), (float)acc[par][i]);
+            if (p < P2) put_four1<NT, COH>(four + geo.at(p, m, par, col), (float)acc[par][i]);
           }
       }
     }
+  }
+}
+
+// One column tile per workgroup with at most two K slices (N <= 127) asks
+// the compiler for 5 waves per SIMD, so a T85 batch of 97 fields (1144
+// workgroups) is resident at once (measured at T85: neutral at 97 fields, 6 %
+// faster at 388; the six-product form spills two registers for it).  With
+// more slices the hint would spill heavily.
+template <int MODE, int NSL, int CT, bool NT, int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu((CT == 1 && NSL <= 2) ? 5 : 1))) void
+leg_inv_mixed_kernel(
+    const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
+    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
+    int mlen, int N, int P2, int P2p, int ncol, int nct, FourGeom geo) {
+  __shared__ __align__(16) unsigned char lds[InvItemLds<MODE, NSL, CT, NW>::bytes];
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + CT - 1) / CT, m, t)) return;
+  leg_inv_mixed_item<MODE, NSL, CT, NT, NW, false>(lds, spec, four, L, loff, lexp, N, P2, P2p, ncol, nct,
+                                                   geo, m, t, threadIdx.x);
+}
+
+// ---- Fused inverse: Legendre step and FFT in one persistent launch.
+//
+// The inverse Legendre items (m, 16-column tile) of leg_inv_mixed_kernel and
+// the FFT tiles (latitude pair, field group) of fft_inv_kernel run in one
+// launch, so a call pays one kernel start and one ramp-up of the chip instead
+// of two.  Each workgroup serves the queue of its XCD (blockIdx % 8): first
+// the Legendre items of the wavenumbers dealt to that XCD (column tile by
+// column tile, so the first column tiles complete first), then every eighth
+// FFT tile in field-group order.  A Legendre item stores its part of the
+// Fourier intermediate device-coherently (agent-scope atomic stores), waits
+// for the stores to complete, and counts itself done on its column tile; an
+// FFT tile starts when the column tiles of its field group count all N + 1
+// wavenumbers, and reads the intermediate device-coherently.  A workgroup
+// waiting on a count takes Legendre items no workgroup has taken yet, from
+// any queue; Legendre items never wait, so every count completes whatever
+// the number of workgroups the device runs at once.  A wait beyond one
+// second (a broken device or counter state) gives up: the tile's grid rows
+// are set to NaN and the sync area's error word counts it.  The last
+// workgroup to finish resets the counters for the next call; calls on one
+// plan must therefore not overlap (as for the plan's Fourier scratch).
+struct FusedInv {
+  const double *spec;
+  float *four;
+  double *grid;
+  const uint16_t *L;
+  const int *loff, *lexp, *mlist;
+  const double2 *tw;
+  const double *coslat;
+  int *sync;
+  int mlen, N, P2, P2p, ncol, nct, nf, jm;
+  FourGeom geo;
+};
+
+// Sync area: every counter in a 256-byte slot of its own (kSyncStride ints),
+// so the polls of waiting workgroups and the queue atomics do not share
+// cache lines: slots 0-7 Legendre queues, 8-15 FFT queues, 16 finished
+// workgroups, 17 errors, kFusedSyncHead + c column tile c's count.
+constexpr int kSyncL = 0, kSyncF = 8, kSyncDone = 16, kSyncErr = 17;
+constexpr uint64_t kFusedSpinTicks = 100000000;  // s_memrealtime runs at 100 MHz: one second
+
+__device__ __forceinline__ int sync_load(const int *a) {
+  return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int sync_add(int *a, int v) {
+  return __hip_atomic_fetch_add(a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Three waves per SIMD (three workgroups per CU; the LDS would hold four, the
+// registers of the larger items not).
+template <int L, int MODE, int NSL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void inv_fused_kernel(FusedInv a) {
+  using Lds = InvItemLds<MODE, NSL, 1, 4>;
+  constexpr size_t kFft = sizeof(double2) * kLdsPadded;
+  __shared__ __align__(16) unsigned char lds[Lds::bytes > kFft ? Lds::bytes : kFft];
+  __shared__ double2 twl[L];
+  __shared__ int s_item, s_bad;
+  constexpr int kDone = -1;
+  const int tid = threadIdx.x, x = blockIdx.x & 7;
+  int *const sy = a.sync;
+  auto slot = [&](int k) { return sy + k * kSyncStride; };
+  const int nL = a.mlen * a.nct;                // Legendre items per queue
+  const int nF = a.P2 * a.geo.nfg, nFx = (nF - x + 7) >> 3;
+  const int need = a.N + 1;
+  for (int q = tid; q < L; q += 256) twl[q] = a.tw[q];
+  // Next item, chosen by thread 0: >= 0 Legendre item (queue << 24 | index),
+  // <= -2 FFT tile -2 - tile, kDone.  Thread 0 keeps its queue state.
+  bool own_done = false, all_taken = false;
+  int pend = -1;
+  auto take_l = [&](int y) -> int {  // the next Legendre item of queue y, or -1
+    for (;;) {
+      const int i = sync_add(slot(kSyncL + y), 1);
+      if (i >= nL) return -1;
+      if (a.mlist[y * a.mlen + i % a.mlen] >= 0) return y << 24 | i;
+    }
+  };
+  // a Legendre item of another queue that no workgroup has taken (one poll
+  // of the eight queues, issued together), or -1
+  auto steal = [&]() -> int {
+    int v[8];
+#pragma unroll
+    for (int y = 0; y < 8; ++y) v[y] = sync_load(slot(kSyncL + y));
+    bool any = false;
+#pragma unroll
+    for (int y = 0; y < 8; ++y) any = any || v[y] < nL;
+    if (!any) {
+      all_taken = true;
+      return -1;
+    }
+    for (int k = 1; k <= 8; ++k) {
+      const int y = (x + k) & 7;
+      if (v[y] < nL) {
+        const int it = take_l(y);
+        if (it >= 0) return it;
+      }
+    }
+    return -1;
+  };
+  auto next_item = [&](int &bad) -> int {
+    bad = 0;
+    if (pend < 0) {
+      if (!own_done) {
+        const int it = take_l(x);
+        if (it >= 0) return it;
+        own_done = true;
+      }
+      const int i = sync_add(slot(kSyncF + x), 1);
+      if (i >= nFx) return kDone;
+      pend = x + 8 * i;
+    }
+    const int fgr = pend / a.P2;
+    const int c0 = (fgr << a.geo.lg) >> 4;
+    const int c1 = min((((fgr + 1) << a.geo.lg) - 1) >> 4, a.nct - 1);
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    for (int poll = 0;; ++poll) {
+      // at most 4 column tiles per field group (2G <= 64 columns)
+      int cnt[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) cnt[k] = c0 + k <= c1 ? sync_load(slot(kFusedSyncHead + c0 + k)) : need;
+      bool ready = cnt[0] >= need && cnt[1] >= need && cnt[2] >= need && cnt[3] >= need;
+      if (!ready) {
+        if (!all_taken && poll % 8 == 0) {
+          const int it = steal();
+          if (it >= 0) return it;
+        }
+        if (__builtin_amdgcn_s_memrealtime() - t0 > kFusedSpinTicks) {
+          sync_add(slot(kSyncErr), 1);
+          bad = 1;
+          ready = true;
+        }
+      }
+      if (ready) {
+        const int f = pend;
+        pend = -1;
+        return -2 - f;
+      }
+      __builtin_amdgcn_s_sleep(16);
+    }
+  };
+  for (;;) {
+    if (tid == 0) {
+      int bad;
+      s_item = next_item(bad);
+      s_bad = bad;
+    }
+    __syncthreads();
+    const int item = s_item, bad = s_bad;
+    __syncthreads();  // s_item is rewritten only after every wave has read it
+    if (item == kDone) break;
+    // the thread index, opaque to the compiler for this item: otherwise it
+    // hoists both item bodies' index arithmetic out of the loop and keeps it
+    // live throughout (twice the registers of either kernel alone)
+    int ltid = tid;
+    asm volatile("" : "+v"(ltid));
+    if (item >= 0) {
+      const int y = item >> 24, i = item & 0xffffff;
+      const int t = i / a.mlen, m = a.mlist[y * a.mlen + i % a.mlen];
+      leg_inv_mixed_item<MODE, NSL, 1, false, 4, true>(lds, a.spec, a.four, a.L, a.loff, a.lexp, a.N, a.P2,
+                                                       a.P2p, a.ncol, a.nct, a.geo, m, t, ltid);
+      // this thread's coherent stores have completed; after the barrier
+      // every thread's have, and the column tile counts one more m
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0) sync_add(slot(kFusedSyncHead + t), 1);
+    } else {
+      const int tile = -2 - item;
+      fft_inv_tile<L, float, true>(reinterpret_cast<double2 *>(lds), twl, nullptr, a.four, a.grid, 0,
+                                   a.coslat, a.P2, a.N + 1, a.nf, a.jm, tile, ltid);
+      if (bad) {
+        constexpr int G = kLdsComplex / L;
+        const int p = tile % a.P2, f0 = (tile / a.P2) * G;
+        for (int q = tid; q < G * 2 * L; q += 256) {
+          const int g = q / (2 * L), r = q % (2 * L), f = f0 + g;
+          const int j = r < L ? a.P2 + p : a.P2 - 1 - p;
+          if (f < a.nf) a.grid[((size_t)f * a.jm + j) * L + (r % L)] = __builtin_nan("");
+        }
+      }
+    }
+  }
+  if (tid == 0 && sync_add(slot(kSyncDone), 1) == (int)gridDim.x - 1) {
+    // every other workgroup has finished: reset for the next call
+    for (int k = 0; k < kFusedSyncHead + a.nct; ++k)
+      if (k != kSyncErr) __hip_atomic_store(slot(k), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -431,7 +683,7 @@ __global__ __launch_bounds__(NW * 64) void leg_fwd_mixed_kernel(
     const bool v = p < P2 && cc < ncol;
     const float x = four[geo.at(v ? p : 0, m, par, v ? cc : ct0 * 16)];
     return v ? x : 0.0f;
-  });
+  }, threadIdx.x);
   __syncthreads();
   for (int u = wave; u < nunits; u += NW) {
     const int par = u / nrt, rt = u % nrt;
@@ -762,6 +1014,61 @@ int inv_mode(const Plan &p, int shape, const double *spec, float *four, int nf, 
   return fail(SHT_ERR_UNSUPPORTED, "mixed inverse: nmax above 255");
 }
 
+// Fused inverse dispatch: f(L, MODE, NSL) as integral constants for the
+// plan's grid, form and K slices; SHT_ERR_UNSUPPORTED outside the
+// instantiated set (im 64..512, at most 3 slices, i.e. nmax < 192).  As
+// im >= 2 nmax + 2, a grid of L longitudes needs at most L / 128 slices.
+template <class F>
+int fused_dispatch(const Plan &p, F &&f) {
+  using std::integral_constant;
+  const int nsl = roundup(ceil_half(p.N + 1), 32) / 32;
+  auto by_nsl = [&](auto l, auto mode) -> int {
+    constexpr int L = decltype(l)::value;
+    if (nsl == 1) return f(l, mode, integral_constant<int, 1>{});
+    if constexpr (L >= 256)
+      if (nsl == 2) return f(l, mode, integral_constant<int, 2>{});
+    if constexpr (L >= 512)
+      if (nsl == 3) return f(l, mode, integral_constant<int, 3>{});
+    return fail(SHT_ERR_UNSUPPORTED, "fused inverse: nmax above 191");
+  };
+  auto by_mode = [&](auto l) -> int {
+    switch (p.prec) {
+      case SHT_PREC_SPLIT_BF16: return by_nsl(l, integral_constant<int, SHT_PREC_SPLIT_BF16>{});
+      case SHT_PREC_SCALED_FP16: return by_nsl(l, integral_constant<int, SHT_PREC_SCALED_FP16>{});
+      case SHT_PREC_SPLIT_BF16X6: return by_nsl(l, integral_constant<int, SHT_PREC_SPLIT_BF16X6>{});
+    }
+    return fail(SHT_ERR_UNSUPPORTED, "fused inverse: mixed-precision plans only");
+  };
+  switch (p.im) {
+    case 64: return by_mode(integral_constant<int, 64>{});
+    case 128: return by_mode(integral_constant<int, 128>{});
+    case 256: return by_mode(integral_constant<int, 256>{});
+    case 512: return by_mode(integral_constant<int, 512>{});
+  }
+  return fail(SHT_ERR_UNSUPPORTED, "fused inverse: im must be 64 to 512");
+}
+
+bool fused_supported(const Plan &p) {
+  return p.prec != SHT_PREC_FP64 && p.im >= 64 && p.im <= 512 && ceil_half(p.N + 1) <= 96;
+}
+
+// Sync area and residency of the fused inverse (plan creation).
+int build_fused(Plan &p) {
+  if (!fused_supported(p)) return SHT_OK;
+  const size_t n = (kFusedSyncHead + (size_t)(2 * p.kmax + 15) / 16) * kSyncStride;
+  SHT_PLAN_ALLOC(p, &p.fsync, sizeof(int) * n);
+  SHT_HIP(hipMemset(p.fsync, 0, sizeof(int) * n));
+  int ncu = 0;
+  SHT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, p.device));
+  return fused_dispatch(p, [&](auto l, auto mode, auto nsl) -> int {
+    int per_cu = 0;
+    SHT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, inv_fused_kernel<decltype(l)::value, decltype(mode)::value, decltype(nsl)::value>, 256, 0));
+    p.fused_blocks = per_cu * ncu;
+    return SHT_OK;
+  });
+}
+
 template <int MODE, int NSL>
 int fwd_nsl(const Plan &p, int shape, const float *four, double *spec, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
@@ -806,12 +1113,14 @@ int fwd_mode(const Plan &p, int shape, const float *four, double *spec, int nf, 
 }  // namespace
 
 int build_mixed_tables(Plan &p) {
+  int rc;
   switch (p.prec) {
-    case SHT_PREC_SPLIT_BF16: return build_mixed_mode<SHT_PREC_SPLIT_BF16>(p);
-    case SHT_PREC_SCALED_FP16: return build_mixed_mode<SHT_PREC_SCALED_FP16>(p);
-    case SHT_PREC_SPLIT_BF16X6: return build_mixed_mode<SHT_PREC_SPLIT_BF16X6>(p);
+    case SHT_PREC_SPLIT_BF16: rc = build_mixed_mode<SHT_PREC_SPLIT_BF16>(p); break;
+    case SHT_PREC_SCALED_FP16: rc = build_mixed_mode<SHT_PREC_SCALED_FP16>(p); break;
+    case SHT_PREC_SPLIT_BF16X6: rc = build_mixed_mode<SHT_PREC_SPLIT_BF16X6>(p); break;
+    default: return fail(SHT_ERR_INVALID, "build_mixed_tables: not a mixed-precision plan");
   }
-  return fail(SHT_ERR_INVALID, "build_mixed_tables: not a mixed-precision plan");
+  return rc ? rc : build_fused(p);
 }
 
 int launch_leg_inv_mixed_shape(const Plan &p, int shape, const double *spec, float *four, int nf,
@@ -836,6 +1145,41 @@ int launch_leg_fwd_mixed_shape(const Plan &p, int shape, const float *four, doub
 
 int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s) {
   return launch_leg_inv_mixed_shape(p, resolve_shape(p, nf, 0), spec, four, nf, s);
+}
+
+bool fused_inv_available(const Plan &p) { return fused_supported(p) && p.fsync && p.fused_blocks >= 8; }
+
+int launch_inv_fused(const Plan &p, const double *spec, float *four, double *grid, int nf, hipStream_t s) {
+  if (!fused_inv_available(p)) return fail(SHT_ERR_UNSUPPORTED, "fused inverse: not available for this plan");
+  FusedInv a{};
+  a.spec = spec;
+  a.four = four;
+  a.grid = grid;
+  a.L = static_cast<const uint16_t *>(p.Linv);
+  a.loff = p.loff_inv;
+  a.lexp = p.lexp_inv;
+  a.mlist = p.xcd_m[0];
+  a.tw = reinterpret_cast<const double2 *>(p.twid);
+  a.coslat = p.coslat;
+  a.sync = p.fsync;
+  a.mlen = p.xcd_mlen[0];
+  a.N = p.N;
+  a.P2 = p.P2;
+  a.P2p = p.P2p;
+  a.ncol = 2 * nf;
+  a.nct = (a.ncol + 15) / 16;
+  a.nf = nf;
+  a.jm = p.jm;
+  a.geo = four_geom_out(p, nf, sizeof(float));
+  // no more workgroups than items, a multiple of 8 (one queue per XCD)
+  const long items = 8L * a.mlen * a.nct + (long)p.P2 * a.geo.nfg;
+  const int nblk = (int)std::max(8L, std::min<long>(p.fused_blocks, items) / 8 * 8);
+  return fused_dispatch(p, [&](auto l, auto mode, auto nsl) -> int {
+    launch_kernel(inv_fused_kernel<decltype(l)::value, decltype(mode)::value, decltype(nsl)::value>, nblk, 256,
+                  0, s, a);
+    SHT_HIP(hipGetLastError());
+    return SHT_OK;
+  });
 }
 
 int launch_leg_fwd_mixed(const Plan &p, const float *four, double *spec, int nf, hipStream_t s) {

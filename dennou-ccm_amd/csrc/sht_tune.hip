@@ -17,12 +17,16 @@
 namespace sht {
 namespace {
 
-// FP64 plans: every shape; mixed-precision plans: KSPLIT (shared LDS data
-// tile of one column tile), WAVE (data tile per wave in registers) and TILE
-// (shared LDS data tiles of two column tiles).
-int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_NSHAPE : 3; }
+// FP64 plans: every shape but FUSED; mixed-precision plans: KSPLIT (shared
+// LDS data tile of one column tile), WAVE (data tile per wave in registers)
+// and TILE (shared LDS data tiles of two column tiles), and for the inverse
+// FUSED where the plan has it.
+int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_SHAPE_STAGED + 1 : 3; }
 
-bool shape_ok(const Plan &p, int s) { return s == SHT_SHAPE_AUTO || (s >= 0 && s < nshapes(p)); }
+bool shape_ok(const Plan &p, int s, int dir) {
+  if (s == SHT_SHAPE_FUSED) return dir == 0 && fused_inv_available(p);
+  return s == SHT_SHAPE_AUTO || (s >= 0 && s < nshapes(p));
+}
 
 host::Tuning *find_entry(Plan &p, int nf) {
   for (Plan::ShapeEntry &e : p.shapes)
@@ -97,6 +101,7 @@ struct Candidate {
 std::vector<Candidate> candidates(const Plan &p, int dir, bool classes) {
   std::vector<Candidate> c;
   for (int s = 0; s < nshapes(p); ++s) c.push_back({s, KSplit{-1, -1}});
+  if (dir == 0 && fused_inv_available(p)) c.push_back({SHT_SHAPE_FUSED, KSplit{-1, -1}});
   if (!classes || p.prec != SHT_PREC_FP64 || p.N < 8) return c;
   const KSplit def = default_classes(p, dir);
   for (int pct = 20; pct <= 70; pct += 10) {
@@ -123,7 +128,7 @@ int sht_plan_set_shapes(sht_plan *h, int nf, int inv, int fwd) {
   if (!h) return fail(SHT_ERR_INVALID, "sht_plan_set_shapes: plan is NULL");
   Plan &p = h->p;
   if (nf < 0 || nf > p.kmax) return fail(SHT_ERR_INVALID, "sht_plan_set_shapes: nf out of range");
-  if (!shape_ok(p, inv) || !shape_ok(p, fwd))
+  if (!shape_ok(p, inv, 0) || !shape_ok(p, fwd, 1))
     return fail(SHT_ERR_UNSUPPORTED, "sht_plan_set_shapes: shape not available for this precision");
   if (nf == 0) {
     p.shape_inv = inv;
@@ -194,7 +199,7 @@ int sht_plan_autotune_flags(sht_plan *h, int nf, int reps, const char *cache_pat
     auto it = cache.find(key);
     if (it != cache.end()) {
       const host::Tuning &c = it->second;
-      if (shape_ok(p, c.inv) && shape_ok(p, c.fwd) && c.inv >= 0 && c.fwd >= 0 &&
+      if (shape_ok(p, c.inv, 0) && shape_ok(p, c.fwd, 1) && c.inv >= 0 && c.fwd >= 0 &&
           classes_ok(p, c.inv_m4, c.inv_m2) && classes_ok(p, c.fwd_m4, c.fwd_m2)) {
         *find_entry(p, nf) = c;
         return SHT_OK;
@@ -232,7 +237,9 @@ int sht_plan_autotune_flags(sht_plan *h, int nf, int reps, const char *cache_pat
           float *f4 = reinterpret_cast<float *>(p.four);
           rc = time_launches(s, reps, [&] {
             int r;
-            if (dir == 0) {
+            if (dir == 0 && c.shape == SHT_SHAPE_FUSED) {
+              r = launch_inv_fused(p, spec, f4, grid, nf, s);
+            } else if (dir == 0) {
               r = f32 ? launch_leg_inv_mixed_shape(p, c.shape, spec, f4, nf, s)
                       : launch_leg_inv_f64_shape(p, c.shape, spec, p.four, p.four2, true, false, nf, s, c.cls);
               if (!r) r = launch_fft_inv(p, p.four, nullptr, f32, grid, nullptr, nullptr, nf, s);

@@ -27,8 +27,12 @@ SHT_ERR_BUSY = -5
 PREC_FP64, PREC_SPLIT_BF16, PREC_SCALED_FP16, PREC_SPLIT_BF16X6 = 0, 1, 2, 3
 DIR_INVERSE, DIR_FORWARD, DIR_ROUNDTRIP = 0, 1, 2
 SHAPE_AUTO, SHAPE_KSPLIT, SHAPE_WAVE, SHAPE_TILE, SHAPE_TILE_SPLIT, SHAPE_STAGED = -1, 0, 1, 2, 3, 4
-SHAPES = ("ksplit", "wave", "tile", "tile_split", "staged")
+SHAPES = ("ksplit", "wave", "tile", "tile_split", "staged")  # the FP64 plans' shapes
 NSHAPE = len(SHAPES)
+# inverse of mixed-precision plans (64 <= im <= 512, nmax < 192): Legendre
+# step and ring FFTs in one persistent launch
+SHAPE_FUSED = 5
+SHAPE_NAMES = SHAPES + ("fused",)
 FLAG_GRAD = 1
 TUNE_SHAPES_ONLY = 1
 

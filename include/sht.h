@@ -347,15 +347,20 @@ enum {
   SHT_SHAPE_TILE_SPLIT = 3,   /* TILE with the K loop halved for the heaviest m         */
   SHT_SHAPE_STAGED = 4,       /* workgroup per (m, 16 columns), data block in LDS, each
                                  wave a (parity, row tile) over the whole K (FP64 only)  */
-  SHT_NSHAPE = 5
+  SHT_SHAPE_FUSED = 5,        /* inverse of mixed plans with 64 <= im <= 512 and
+                                 nmax < 192: KSPLIT's Legendre items and the ring FFTs
+                                 in one persistent launch (grid values; gradients and
+                                 Legendre-only launches run KSPLIT)                      */
+  SHT_NSHAPE = 6
 };
 /* Fixes the shapes for calls with nf fields (nf = 0: the plan default for every
- * count without an entry).  FP64 plans take every shape.  Mixed-precision
+ * count without an entry).  FP64 plans take every shape but SHT_SHAPE_FUSED.  Mixed-precision
  * plans take SHT_SHAPE_AUTO, SHT_SHAPE_KSPLIT (workgroup per (m, 16 columns),
  * data tile shared in LDS), SHT_SHAPE_WAVE (wavefront per 16 columns, data
  * tile in registers) and SHT_SHAPE_TILE (workgroup per (m, 32 columns), each
- * wave's table fragments used for both column tiles); their results are
- * bitwise equal. */
+ * wave's table fragments used for both column tiles), and for the inverse
+ * SHT_SHAPE_FUSED where the plan has it (SHT_ERR_UNSUPPORTED otherwise);
+ * their results are bitwise equal. */
 int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
 /* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
 int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);
