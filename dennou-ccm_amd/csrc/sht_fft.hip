@@ -67,7 +67,11 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
   const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const int tid = threadIdx.x;
-  for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
+  // the twiddles are loaded after the first ring loads are issued (one wait
+  // covers both; loaded first, their wait held back every ring load)
+  auto twiddles = [&] {
+    for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
+  };
   // z = f_N + i f_S; transforms past nf are zero
   auto ring = [&](int g, int i) {
     const int f = f0 + g;
@@ -82,7 +86,7 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
     if (i < M1 || i > L - M1) buf[px<L>(g * L + i)] = v;
   };
   if constexpr (DIRECT) {
-    fft_lds<L, -1>(buf, twl, ring, keep, threadIdx.x);
+    fft_lds<L, -1>(buf, twl, ring, keep, threadIdx.x, twiddles);
   } else {
     constexpr int NQ = kLdsComplex / kThreads;  // samples per thread
     double2 z[NQ];
@@ -91,6 +95,7 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
       const int idx = tid + q * kThreads;
       z[q] = ring(idx / L, idx % L);
     }
+    twiddles();
 #pragma unroll
     for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = z[q];
     __syncthreads();

@@ -98,20 +98,33 @@ __device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
 // multiplication from one LDS twiddle), runs a register DFT and writes
 // y[(j/Ns) Ns R + k + r Ns] through out(g, i, v).  A barrier separates the
 // reads from the writes (the passes run in place in buf) and follows them.
-template <int L, int R, int Ns, int SIGN, class In, class Out>
-__device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out, int tid) {
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
+template <int L, int R, int Ns, int SIGN, class In, class Out, class Hook = NoHook>
+__device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out, int tid, Hook hook = {}) {
 #pragma clang fp contract(off)
   constexpr int G = kLdsComplex / L;
   constexpr int NB = G * (L / R);                       // butterflies in the block
   constexpr int QB = (NB + kThreads - 1) / kThreads;    // per thread
   double2 v[QB][R];
+  // all of the pass's reads first, then hook() (work whose own loads then
+  // share one wait with them), then the butterflies
 #pragma unroll
   for (int q = 0; q < QB; ++q) {
     const int idx = tid + q * kThreads;
     if (NB % kThreads != 0 && idx >= NB) continue;
-    const int g = idx / (L / R), j = idx % (L / R), k = j % Ns;
+    const int g = idx / (L / R), j = idx % (L / R);
 #pragma unroll
     for (int r = 0; r < R; ++r) v[q][r] = in(g, j + r * (L / R));
+  }
+  hook();
+#pragma unroll
+  for (int q = 0; q < QB; ++q) {
+    const int idx = tid + q * kThreads;
+    if (NB % kThreads != 0 && idx >= NB) continue;
+    const int j = idx % (L / R), k = j % Ns;
     if constexpr (Ns > 1) {
       double2 w1 = tw[k * (L / (Ns * R))];
       w1.y *= SIGN;
@@ -145,36 +158,38 @@ __device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out,
 // save the LDS round trips; the passes between run in place in buf.
 // Radix plan: 16 = 16, 32 = 8*4, 64 = 8*8, 128 = 16*8, 256 = 16*16,
 // 512 = 8*8*8, 1024 = 16*16*4, 2048 = 16*16*8.  Ends with a barrier.
-template <int L, int SIGN, class First, class Last>
-__device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw, First first, Last last, int tid) {
+// hook() runs in the first pass once its reads are issued (see stockham_pass).
+template <int L, int SIGN, class First, class Last, class Hook = NoHook>
+__device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw, First first, Last last, int tid,
+                                        Hook hook = {}) {
 #pragma clang fp contract(off)
   auto rd = [&](int g, int i) { return buf[px<L>(g * L + i)]; };
   auto wr = [&](int g, int i, double2 v) { buf[px<L>(g * L + i)] = v; };
   if constexpr (L == 16) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, last, tid);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, last, tid, hook);
   } else if constexpr (L == 32) {
-    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 4, 8, SIGN>(tw, rd, last, tid);
   } else if constexpr (L == 64) {
-    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 8, 8, SIGN>(tw, rd, last, tid);
   } else if constexpr (L == 128) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 8, 16, SIGN>(tw, rd, last, tid);
   } else if constexpr (L == 256) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 16, 16, SIGN>(tw, rd, last, tid);
   } else if constexpr (L == 512) {
-    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 8, 8, SIGN>(tw, rd, wr, tid);
     stockham_pass<L, 8, 64, SIGN>(tw, rd, last, tid);
   } else if constexpr (L == 1024) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 16, 16, SIGN>(tw, rd, wr, tid);
     stockham_pass<L, 4, 256, SIGN>(tw, rd, last, tid);
   } else {
     static_assert(L == 2048, "unsupported ring length");
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid);
+    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
     stockham_pass<L, 16, 16, SIGN>(tw, rd, wr, tid);
     stockham_pass<L, 8, 256, SIGN>(tw, rd, last, tid);
   }

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_kernel(const double *__restri
       const int k = r * 16 + i * 4 + (ln >> 4), c = ct * 16 + (ln & 15);
       if (k < kcnt[q] && c < ncol) {
         double *o = sm + (size_t)(2 * k + q) * ncol + c;
-        __builtin_nontemporal_store(accum ? *o + v : v, o);
+        put_spec(o, accum ? *o + v : v);
       }
     }
     __syncthreads();
@@ -333,8 +333,8 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_wave_kernel(
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int k = kt * 16 + i * 4 + kq;
-      if (k < ke) __builtin_nontemporal_store(acc0[i], sm + (size_t)(2 * k) * ncol);
-      if (k < ko) __builtin_nontemporal_store(acc1[i], sm + (size_t)(2 * k + 1) * ncol);
+      if (k < ke) put_spec(sm + (size_t)(2 * k) * ncol, acc0[i]);
+      if (k < ko) put_spec(sm + (size_t)(2 * k + 1) * ncol, acc1[i]);
     }
   }
 }
@@ -589,8 +589,8 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_tile_kernel(
 #pragma unroll
           for (int i = 0; i < 4; ++i) {
             const int k = r * 16 + i * 4 + kq;
-            if (k < ke) __builtin_nontemporal_store(acc[0][r][i], sm + (size_t)(2 * k) * ncol + col);
-            if (k < ko) __builtin_nontemporal_store(acc[1][r][i], sm + (size_t)(2 * k + 1) * ncol + col);
+            if (k < ke) put_spec(sm + (size_t)(2 * k) * ncol + col, acc[0][r][i]);
+            if (k < ko) put_spec(sm + (size_t)(2 * k + 1) * ncol + col, acc[1][r][i]);
           }
     }
     return;
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(256) void leg_fwd_f64_tile_kernel(
       const int ln = el & 63, ri = el >> 6, i = ri & 3, r = ri >> 2;
       const int k = r * 16 + i * 4 + (ln >> 4);
       const int c = (ctg * cpw + g) * 16 + (ln & 15);
-      if (k < kcnt[q] && c < ncol) __builtin_nontemporal_store(v, sm + (size_t)(2 * k + q) * ncol + c);
+      if (k < kcnt[q] && c < ncol) put_spec(sm + (size_t)(2 * k + q) * ncol + c, v);
     }
     __syncthreads();
   }
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(NW * 64) void leg_fwd_f64_staged_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = rt * 16 + i * 4 + kq;
-          if (k < kn) __builtin_nontemporal_store(acc[c][i], sm + (size_t)(2 * k + q) * ncol + col);
+          if (k < kn) put_spec(sm + (size_t)(2 * k + q) * ncol + col, acc[c][i]);
         }
     }
   }
@@ -916,7 +916,7 @@ __global__ __launch_bounds__(NW * 64) void leg_fwd_f64_halves_kernel(
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int k = rt * 16 + j * 4 + kq;
-          if (k < kn) __builtin_nontemporal_store(acc[i][c][j], sm + (size_t)(2 * k + q) * ncol + col);
+          if (k < kn) put_spec(sm + (size_t)(2 * k + q) * ncol + col, acc[i][c][j]);
         }
     }
   }

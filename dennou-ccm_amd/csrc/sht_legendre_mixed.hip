@@ -712,7 +712,7 @@ __global__ __launch_bounds__(NW * 64) void leg_fwd_mixed_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = rt * 16 + kq * 4 + i;
-          if (k < kc) __builtin_nontemporal_store(acc[i], sj + (size_t)(2 * k + par) * ncol);
+          if (k < kc) put_spec(sj + (size_t)(2 * k + par) * ncol, acc[i]);
         }
       }
     }
@@ -883,7 +883,7 @@ __global__ __launch_bounds__(256) void leg_fwd_mixed_wave_kernel(
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = kt * 16 + kq * 4 + i;
-          if (k < kc) __builtin_nontemporal_store(acc[i], sm + (size_t)(2 * k + par) * ncol);
+          if (k < kc) put_spec(sm + (size_t)(2 * k + par) * ncol, acc[i]);
         }
       }
     }
