@@ -196,22 +196,22 @@ __device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw, First f
 }
 
 
-// Loads of the Fourier intermediate: COH = device-coherent (agent-scope
-// relaxed atomic loads, 8 bytes each), for data another workgroup of the same
-// kernel wrote; plain loads otherwise.
+// Loads of the Fourier intermediate, one (re, im) pair in its stored type
+// (float2 or double2): COH = device-coherent (agent-scope relaxed atomic
+// loads, 8 bytes each), for data another workgroup of the same kernel wrote;
+// plain loads otherwise.  The pair is 8- or 16-byte aligned.
 template <bool COH>
-__device__ __forceinline__ double2 load_four_pair(const float *e) {
+__device__ __forceinline__ float2 load_four_raw(const float *e) {
   if constexpr (COH) {
     const unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long *>(e),
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const float2 v = __builtin_bit_cast(float2, u);
-    return make_double2((double)v.x, (double)v.y);
+    return __builtin_bit_cast(float2, u);
   } else {
-    return make_double2((double)e[0], (double)e[1]);
+    return *reinterpret_cast<const float2 *>(e);
   }
 }
 template <bool COH>
-__device__ __forceinline__ double2 load_four_pair(const double *e) {
+__device__ __forceinline__ double2 load_four_raw(const double *e) {
   if constexpr (COH) {
     const unsigned long long a = __hip_atomic_load(reinterpret_cast<const unsigned long long *>(e),
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -219,9 +219,11 @@ __device__ __forceinline__ double2 load_four_pair(const double *e) {
                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return make_double2(__builtin_bit_cast(double, a), __builtin_bit_cast(double, b));
   } else {
-    return make_double2(e[0], e[1]);
+    return *reinterpret_cast<const double2 *>(e);
   }
 }
+__device__ __forceinline__ double2 widen(float2 v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ double2 widen(double2 v) { return v; }
 
 // One inverse tile: four (even/odd Legendre sums) -> grid for latitude pair
 // p and field group fgr (tile = fgr * P2 + p).  A thread's Fourier loads (at
@@ -244,17 +246,20 @@ __device__ __forceinline__ void fft_inv_tile(double2 *buf, double2 *twl, const d
   const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const double rc = (mode == 0) ? 1.0 : 1.0 / coslat[jn];
-  double2 E[NM], O[NM];
+  // every load issued unconditionally (pairs past M1 or nf read the tile's
+  // first pair and are not used), converted to FP64 only after the
+  // twiddles: converting inside a branch per pair made the FP32 form wait
+  // for each pair before issuing the next
+  using Raw = decltype(load_four_raw<COH>(src));
+  Raw Er[NM], Or[NM];
 #pragma unroll
   for (int q = 0; q < NM; ++q) {
     const int idx = tid + q * kThreads;
     const int m = idx / G, g = idx % G, f = f0 + g;
-    E[q] = O[q] = make_double2(0.0, 0.0);
-    if (m < M1 && f < nf) {
-      const TF *e = src + ((((size_t)p * nfg + fgr) * M1 + m) * 2) * (2 * G) + 2 * g;
-      E[q] = load_four_pair<COH>(e);
-      O[q] = load_four_pair<COH>(e + 2 * G);
-    }
+    const bool ok = m < M1 && f < nf;
+    const TF *e = src + ((((size_t)p * nfg + fgr) * M1 + (ok ? m : 0)) * 2) * (2 * G) + (ok ? 2 * g : 0);
+    Er[q] = load_four_raw<COH>(e);
+    Or[q] = load_four_raw<COH>(e + 2 * G);
   }
   if (tw)
     for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];
@@ -263,7 +268,8 @@ __device__ __forceinline__ void fft_inv_tile(double2 *buf, double2 *twl, const d
     const int idx = tid + q * kThreads;
     const int m = idx / G, g = idx % G, f = f0 + g;
     if (m >= M1 || f >= nf) continue;
-    double2 xn = cadd(E[q], O[q]), xs = csub(E[q], O[q]);
+    const double2 Eq = widen(Er[q]), Oq = widen(Or[q]);
+    double2 xn = cadd(Eq, Oq), xs = csub(Eq, Oq);
     if (mode == 1) { // i m / cos(lat)
       const double s = m * rc;
       xn = make_double2(-xn.y * s, xn.x * s);
