@@ -149,16 +149,16 @@ __device__ __forceinline__ void put_four(T *dst, T v) {
   else
     *dst = v;
 }
-// Store of forward Legendre results (spectra, rows of 2 nf doubles).  A
-// workgroup writes 16-column strips of its rows; the row pitch is no multiple
-// of a 128-byte line, so a strip covers parts of two lines whose other parts
-// belong to the neighbouring column tiles, which run on the same XCD (the
-// blocks of one m are dealt to one XCD).  Regular (write-back) stores let
-// that L2 join the parts, so HBM receives whole lines: measured at T85 with
-// 97 fields, 5.81 MB written per dispatch (the spectra's own size) against
-// 7.4-8.1 MB with streaming stores, for about +0.25 us per launch (the
-// write-back at the kernel's end).
-__device__ __forceinline__ void put_spec(double *dst, double v) { *dst = v; }
+// Store of forward Legendre results (spectra, rows of 2 nf doubles):
+// streaming.  A workgroup writes 16-column strips of its rows; the row pitch
+// is no multiple of a 128-byte line, so a strip covers parts of two lines
+// and HBM receives partial lines (T85, 97 fields: 7.7-8.0 MB written per
+// dispatch for 5.81 MB of spectra).  Regular write-back stores let the XCD's
+// L2 join the parts (all column strips of one m run on one XCD) and write
+// exactly 5.81 MB, but the write-back at the kernel's end delays the next
+// launch: the T85 round trip measured 40.15 against 39.40 us (split-BF16
+// x6) and 46.45 against 46.17 us (FP64), so the stores stream.
+__device__ __forceinline__ void put_spec(double *dst, double v) { __builtin_nontemporal_store(v, dst); }
 inline int fft_group(int im) { return 2048 / im; }
 inline FourGeom four_geom(int N, int im, int nf) {
   const int G = fft_group(im);
