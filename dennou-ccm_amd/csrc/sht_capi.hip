@@ -32,6 +32,8 @@ static void release(Plan &p) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   if (p.pinned) (void)hipHostFree(p.pinned);
+  for (hipEvent_t e : p.host_events)
+    if (e) (void)hipEventDestroy(e);
   if (p.host_stream) (void)hipStreamDestroy(p.host_stream);
   p = Plan{};
 }
@@ -55,6 +57,44 @@ static int ensure_host_staging(Plan &p) {
   SHT_HIP(hipMalloc(&p.stage_b, sizeof(double) * spec));
   SHT_HIP(hipHostMalloc(&p.pinned, sizeof(double) * big, hipHostMallocDefault));
   p.pinned_bytes = sizeof(double) * big;
+  for (hipEvent_t &e : p.host_events) SHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  return SHT_OK;
+}
+
+// The host copies of a drop-in call, in chunks of at least kHostChunkBytes
+// (at most 8) so that the PCIe copy of one chunk overlaps the host copy of
+// another: in, each chunk's host -> pinned copy is followed by its async
+// upload; out, the downloads are queued behind the transform and each
+// chunk's pinned -> host copy starts when its download has landed.
+constexpr size_t kHostChunkBytes = (size_t)2 << 20;
+
+static int host_chunks(size_t bytes) {
+  const size_t n = bytes / kHostChunkBytes;
+  return n < 1 ? 1 : n > 8 ? 8 : (int)n;
+}
+
+static int upload_chunked(Plan &p, const double *in, size_t n, hipStream_t s) {
+  const int nc = host_chunks(sizeof(double) * n);
+  for (int c = 0; c < nc; ++c) {
+    const size_t a = n * c / nc, b = n * (c + 1) / nc;
+    std::memcpy(p.pinned + a, in + a, sizeof(double) * (b - a));
+    SHT_HIP(hipMemcpyAsync(p.stage_a + a, p.pinned + a, sizeof(double) * (b - a), hipMemcpyHostToDevice, s));
+  }
+  return SHT_OK;
+}
+
+static int download_chunked(Plan &p, double *out, size_t n, hipStream_t s) {
+  const int nc = host_chunks(sizeof(double) * n);
+  for (int c = 0; c < nc; ++c) {
+    const size_t a = n * c / nc, b = n * (c + 1) / nc;
+    SHT_HIP(hipMemcpyAsync(p.pinned + a, p.stage_a + a, sizeof(double) * (b - a), hipMemcpyDeviceToHost, s));
+    SHT_HIP(hipEventRecord(p.host_events[c], s));
+  }
+  for (int c = 0; c < nc; ++c) {
+    const size_t a = n * c / nc, b = n * (c + 1) / nc;
+    SHT_HIP(hipEventSynchronize(p.host_events[c]));
+    std::memcpy(out + a, p.pinned + a, sizeof(double) * (b - a));
+  }
   return SHT_OK;
 }
 
@@ -375,38 +415,40 @@ static int host_call(sht_plan *h, const double *in, double *out, int nlev, HostO
   const bool to_spec = op == kWaXya || op == kDivLon || op == kDivLat;
   const size_t nin = to_spec ? ngrid : nwa;
   const size_t nout = to_spec ? nwa : ngrid;
-  std::memcpy(p.pinned, in, sizeof(double) * nin);
-  SHT_HIP(hipMemcpyAsync(p.stage_a, p.pinned, sizeof(double) * nin, hipMemcpyHostToDevice, s));
-  switch (op) {
+  rc = upload_chunked(p, in, nin, s);
+  if (!rc) switch (op) {
     case kXyaWa:
-      if ((rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s))) return rc;
-      if ((rc = spec_to_grid(p, p.stage_b, p.stage_a, nlev, s))) return rc;
+      if (!(rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s)))
+        rc = spec_to_grid(p, p.stage_b, p.stage_a, nlev, s);
       break;
     case kGradLon:
-      if ((rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s))) return rc;
-      if ((rc = spec_to_grad(p, p.stage_b, nullptr, p.stage_a, nullptr, nlev, s))) return rc;
+      if (!(rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s)))
+        rc = spec_to_grad(p, p.stage_b, nullptr, p.stage_a, nullptr, nlev, s);
       break;
     case kGradLat:
-      if ((rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s))) return rc;
-      if ((rc = spec_to_grad(p, p.stage_b, nullptr, nullptr, p.stage_a, nlev, s))) return rc;
+      if (!(rc = launch_spml_to_canon(p, p.stage_a, p.stage_b, nlev, s)))
+        rc = spec_to_grad(p, p.stage_b, nullptr, nullptr, p.stage_a, nlev, s);
       break;
     case kWaXya:
-      if ((rc = grid_to_spec(p, p.stage_a, p.stage_b, nlev, s))) return rc;
-      if ((rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s))) return rc;
+      if (!(rc = grid_to_spec(p, p.stage_a, p.stage_b, nlev, s)))
+        rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s);
       break;
     case kDivLon:
     case kDivLat: {
       const double *a = op == kDivLon ? p.stage_a : nullptr;
       const double *b = op == kDivLat ? p.stage_a : nullptr;
-      if ((rc = grid_to_div(p, a, b, p.stage_b, nlev, s))) return rc;
-      if ((rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s))) return rc;
+      if (!(rc = grid_to_div(p, a, b, p.stage_b, nlev, s)))
+        rc = launch_canon_to_spml(p, p.stage_b, p.stage_a, nlev, s);
       break;
     }
   }
-  SHT_HIP(hipMemcpyAsync(p.pinned, p.stage_a, sizeof(double) * nout, hipMemcpyDeviceToHost, s));
-  SHT_HIP(hipStreamSynchronize(s));
-  std::memcpy(out, p.pinned, sizeof(double) * nout);
-  return SHT_OK;
+  if (rc) {
+    // uploads may still read the pinned buffer: let them finish before the
+    // next call refills it
+    (void)hipStreamSynchronize(s);
+    return rc;
+  }
+  return download_chunked(p, out, nout, s);
 }
 
 int sht_xya_wa(sht_plan *h, const double *wa, double *xya, int nlev) {

@@ -102,6 +102,7 @@ struct Plan {
   double *pinned = nullptr;                      // pinned host staging
   size_t pinned_bytes = 0;
   hipStream_t host_stream = nullptr;
+  hipEvent_t host_events[8] = {};                // chunks of the host result copy
   // host copies of the grid
   std::vector<double> h_mu, h_w, h_coslat;
   // stage timers (sht_plan_set_timing)
