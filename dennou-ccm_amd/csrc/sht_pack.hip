@@ -20,8 +20,13 @@ __global__ void spml_to_canon_kernel(const double *__restrict__ wa, double *__re
   if (t >= (int64_t)K * nf) return;
   const int f = (int)(t % nf);
   const int k = (int)(t / nf);
-  // invert k = off(m) + n - m
-  int m = 0;
+  // invert k = off(m) + n - m: off(m) = m (2N + 3 - m) / 2, so the largest m
+  // with off(m) <= k is the floor of the smaller root of m^2 - (2N+3) m + 2k,
+  // corrected by a step where the square root rounds across an integer
+  const double b = 2.0 * N + 3.0;
+  int m = (int)floor(0.5 * (b - sqrt(b * b - 8.0 * (double)k)));
+  m = m < 0 ? 0 : m > N ? N : m;
+  while (m > 0 && coef_off(N, m) > k) --m;
   while (m < N && coef_off(N, m + 1) <= k) ++m;
   const int n = m + (k - (int)coef_off(N, m));
   const double *w = wa + (size_t)f * (N + 1) * (N + 1);
