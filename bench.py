@@ -267,6 +267,92 @@ def run_host_dropin(precision: int, reps: int = 20):
             "note": "host arrays, PCIe copies included; not the bench value"}
 
 
+SMALL_CONFIGS = {  # the other single-GPU configurations: (N, im, jm, levels, gradients)
+    0: (21, 64, 32, 16, False),   # T21L16 round trip
+    1: (42, 128, 64, 26, True),   # T42L26: inverse with GradLon/GradLat, forward of a temperature grid
+}
+
+
+def run_small_config(cfg: int, steps: int, cpu_seconds: float):
+    """Configurations [0] and [1]: GPU FP64 and split-BF16 x6 steps (eager,
+    queue prefilled, events on the stream) and the CPU oracle on the host's
+    cores; errors against the FP64 oracle.  Returns a JSON-ready dict."""
+    import torch
+
+    import oracle
+    import shtamd
+    from shtamd import synth
+
+    n, im, jm, nlev, grads = SMALL_CONFIGS[cfg]
+    nf = 3 * nlev + 1
+    spec_c = synth.state_spectra(n, nlev, cfg=cfg)
+    ref = oracle.Plan(n, im, jm)
+    out = {"workload": f"config [{cfg}]: T{n}L{nlev}, {nf} fields, {im}x{jm} Gaussian grid"
+                       + (", inverse with GradLon/GradLat + forward of a 200-300 K grid" if grads else
+                          ", inverse + forward"), "forms": {}}
+    # the temperature grid on the oracle's Gauss latitudes (the GPU plan's own
+    # agree to ~1e-16); forward of it is part of config [1]'s step
+    temp = synth.temperature_grid(oracle.gauss(jm)[2], im, nf, cfg=cfg) if grads else None
+    for form in ("fp64", "bf16x6"):
+        plan = shtamd.Plan(n, im, jm, nf, precision=PRECISIONS[form],
+                           flags=shtamd.FLAG_GRAD if grads else 0)
+        plan.autotune(nf, reps=10)
+        s_in = torch.from_numpy(shtamd.spec_to_real(spec_c)).cuda()
+        g = [torch.empty((nf, jm, im), dtype=torch.float64, device="cuda") for _ in range(3)]
+        s_out = torch.empty_like(s_in)
+        t_in = torch.from_numpy(np.ascontiguousarray(temp)).cuda() if grads else None
+        stream = torch.cuda.current_stream()
+
+        def step():
+            if grads:
+                plan.spec_to_grad(s_in, g[0], g[1], g[2], nf, stream)
+                plan.grid_to_spec(t_in, s_out, nf, stream)
+            else:
+                plan.spec_to_grid(s_in, g[0], nf, stream)
+                plan.grid_to_spec(g[0], s_out, nf, stream)
+
+        for _ in range(5):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(SPIN_CYCLES)
+        e0.record(stream)
+        for _ in range(steps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        err = {"grid_rel_l2": rel(g[0].cpu().numpy(), ref.spec_to_grid(spec_c))}
+        if grads:
+            err["gradlon_rel_l2"] = rel(g[1].cpu().numpy(), ref.spec_to_grid(spec_c, kind=1))
+            err["gradlat_rel_l2"] = rel(g[2].cpu().numpy(), ref.spec_to_grid(spec_c, kind=2))
+            err["forward_rel_l2"] = rel(shtamd.real_to_spec(s_out.cpu().numpy()), ref.grid_to_spec(temp))
+        else:
+            err["roundtrip_rel_l2"] = rel(shtamd.real_to_spec(s_out.cpu().numpy()), spec_c)
+        out["forms"][form] = {"fields_per_s": round(nf / (ms * 1e-3), 1), "ms_per_step": round(ms, 5),
+                              "shapes": [SHAPE_NAMES[x] for x in plan.get_shapes(nf)], "err_vs_fp64_oracle": err}
+        plan.close()
+
+    def cpu(threads, seconds):
+        t0, reps = time.perf_counter(), 0
+        while True:
+            if grads:
+                for kind in range(3):
+                    ref.spec_to_grid(spec_c, kind=kind, nthreads=threads)
+                ref.grid_to_spec(temp, nthreads=threads)
+            else:
+                ref.grid_to_spec(ref.spec_to_grid(spec_c, nthreads=threads), nthreads=threads)
+            reps += 1
+            dt = time.perf_counter() - t0
+            if dt >= seconds and reps >= 3:
+                return round(reps * nf / dt, 1)
+
+    threads = host_cpu_info()["usable_cpus"]
+    out["cpu_oracle"] = {"fields_per_s": cpu(threads, cpu_seconds), "threads": threads,
+                         "single_core_fields_per_s": cpu(1, max(1.0, cpu_seconds / 4))}
+    return out
+
+
 def copy_calibration(nbytes: int = 64 << 20, reps: int = 3):
     """A plain device copy of known size: under rocprofv3 --pmc FETCH_SIZE and
     --pmc WRITE_SIZE, what the counters read per dispatch against the bytes
@@ -411,7 +497,7 @@ def main():
     ap.add_argument("--precision", default="auto", choices=sorted(PRECISIONS) + ["auto"],
                     help="auto: run every form; the headline is split-BF16 (six products, <=1e-6)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve", "copycal"],
+    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve", "copycal", "small"],
                     help="ensemble: the 64-member mixed-truncation batch; serve: request streams at "
                          "100-1000 req/s (separate JSON lines); copycal: a plain 64 MiB device copy, "
                          "three dispatches, the calibration run for rocprofv3's FETCH_SIZE/WRITE_SIZE")
@@ -430,6 +516,10 @@ def main():
 
     if args.workload == "copycal":
         print(json.dumps(copy_calibration()))
+        return
+    if args.workload == "small":
+        for cfg in SMALL_CONFIGS:
+            print(json.dumps(run_small_config(cfg, args.steps, min(args.cpu_seconds, 5.0))), flush=True)
         return
 
     import oracle
