@@ -256,7 +256,7 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   // failed lookup leaves that entry point reporting it, not plan creation
   (void)noalloc_init_device(p.device);
   rc = build_tables(p);
-  if (!rc) p.fft_fwd_resident = fft_fwd_capacity(p);
+  if (!rc) fft_fwd_capacity(p, p.fft_fwd_resident);
   if (!rc && p.prec != SHT_PREC_FP64) rc = build_mixed_tables(p);
   if (!rc) {
     // [P2][nfg][N+1][2][2G]: field groups of G = 2048/im are allocated whole

@@ -51,16 +51,17 @@ struct FwdIO {
 // workgroups of the last, partly empty field group evenly over the XCDs (T85,
 // 97 fields: inverse 9.8 -> 9.5 us); otherwise each XCD takes a contiguous
 // range of tiles, which measured ~1 % faster when blocks queue.
-template <int L, typename TF, bool DIRECT>
-__global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
-                                                           const double2 *__restrict__ tw,
-                                                           const double *__restrict__ coslat,
-                                                           int P2, int M1, int nf, int jm, int spread) {
+template <int L, typename TF, bool DIRECT, int C>
+__global__ __launch_bounds__(kThreadsOf<C>) void fft_fwd_kernel(FwdIO<TF> io,
+                                                                const double2 *__restrict__ tw,
+                                                                const double *__restrict__ coslat,
+                                                                int P2, int M1, int nf, int jm, int spread) {
   const double *__restrict__ grid = io.src[blockIdx.y];
   TF *__restrict__ four = io.dst[blockIdx.y];
   const int mode = io.mode[blockIdx.y];
-  constexpr int G = kLdsComplex / L;
-  __shared__ double2 buf[kLdsPadded];
+  constexpr int G = C / L;
+  constexpr int kThreads = kThreadsOf<C>;
+  __shared__ double2 buf[kPaddedOf<C>];
   __shared__ double2 twl[L];
   const int tile = spread ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
@@ -86,9 +87,9 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
     if (i < M1 || i > L - M1) buf[px<L>(g * L + i)] = v;
   };
   if constexpr (DIRECT) {
-    fft_lds<L, -1>(buf, twl, ring, keep, threadIdx.x, twiddles);
+    fft_lds<L, -1, C>(buf, twl, ring, keep, threadIdx.x, twiddles);
   } else {
-    constexpr int NQ = kLdsComplex / kThreads;  // samples per thread
+    constexpr int NQ = C / kThreads;  // samples per thread
     double2 z[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(kThreads) void fft_fwd_kernel(FwdIO<TF> io,
 #pragma unroll
     for (int q = 0; q < NQ; ++q) buf[px<L>(tid + q * kThreads)] = z[q];
     __syncthreads();
-    fft_lds<L, -1>(buf, twl, [&](int g, int i) { return buf[px<L>(g * L + i)]; }, keep, threadIdx.x);
+    fft_lds<L, -1, C>(buf, twl, [&](int g, int i) { return buf[px<L>(g * L + i)]; }, keep, threadIdx.x);
   }
   const double sc = 0.5 / L * (mode == 0 ? 1.0 : 1.0 / coslat[jn]);
   for (int idx = tid; idx < M1 * G; idx += kThreads) {
@@ -132,51 +133,66 @@ struct InvOutputs {
 
 // four (even/odd Legendre sums) -> grid.  Blocks: P2 * ceil(nf / G) x noutputs,
 // one tile each (fft_inv_tile).
-template <int L, typename TF>
-__global__ __launch_bounds__(kThreads) void fft_inv_kernel(InvOutputs<TF> io,
-                                                           const double2 *__restrict__ tw,
-                                                           const double *__restrict__ coslat,
-                                                           int P2, int M1, int nf, int jm, int spread) {
-  __shared__ double2 buf[kLdsPadded];
+template <int L, typename TF, int C>
+__global__ __launch_bounds__(kThreadsOf<C>) void fft_inv_kernel(InvOutputs<TF> io,
+                                                                const double2 *__restrict__ tw,
+                                                                const double *__restrict__ coslat,
+                                                                int P2, int M1, int nf, int jm, int spread) {
+  __shared__ double2 buf[kPaddedOf<C>];
   __shared__ double2 twl[L];
   const int tile = spread ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
   const int sel = blockIdx.y;
-  fft_inv_tile<L, TF, false>(buf, twl, tw, io.src[sel], io.dst[sel], io.mode[sel], coslat, P2, M1, nf, jm, tile,
-                             threadIdx.x);
+  fft_inv_tile<L, TF, false, C>(buf, twl, tw, io.src[sel], io.dst[sel], io.mode[sel], coslat, P2, M1, nf, jm,
+                                tile, threadIdx.x);
 }
 
-template <int L, typename TF>
-int fwd_dispatch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
-  constexpr int G = kLdsComplex / L;
+template <int L, int C, typename TF>
+int fwd_launch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
+  constexpr int G = C / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
-  const bool resident = nblk * nin <= p.fft_fwd_resident;
-  auto kern = resident ? fft_fwd_kernel<L, TF, true> : fft_fwd_kernel<L, TF, false>;
-  launch_kernel(kern, dim3(nblk, nin), kThreads, 0, s,
+  const bool resident = nblk * nin <= p.fft_fwd_resident[C == kFftSmall];
+  auto kern = resident ? fft_fwd_kernel<L, TF, true, C> : fft_fwd_kernel<L, TF, false, C>;
+  launch_kernel(kern, dim3(nblk, nin), kThreadsOf<C>, 0, s,
       io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm, resident ? 1 : 0);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
 
-template <int L>
+// the small blocks exist for grids of at most kFftSmallMaxIm longitudes
+template <int L, typename TF>
+int fwd_dispatch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
+  if constexpr (L <= kFftSmallMaxIm)
+    if (fft_block(p, nf) == kFftSmall) return fwd_launch<L, kFftSmall>(p, io, nin, nf, s);
+  return fwd_launch<L, kFftBig>(p, io, nin, nf, s);
+}
+
+template <int L, int C>
 int fwd_resident(int device) {
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_fwd_kernel<L, double, true>, kThreads, 0) !=
-          hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fft_fwd_kernel<L, double, true, C>, kThreadsOf<C>,
+                                                   0) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
     return 0;
   return per_cu * ncu;
 }
 
-template <int L, typename TF>
-int inv_dispatch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
-  constexpr int G = kLdsComplex / L;
+template <int L, int C, typename TF>
+int inv_launch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
+  constexpr int G = C / L;
   const int nblk = p.P2 * ((nf + G - 1) / G);
   // same LDS and block size as the forward kernel, hence the same residency
-  const bool resident = nblk * nout <= p.fft_fwd_resident;
-  launch_kernel(fft_inv_kernel<L, TF>, dim3(nblk, nout), kThreads, 0, s,
+  const bool resident = nblk * nout <= p.fft_fwd_resident[C == kFftSmall];
+  launch_kernel(fft_inv_kernel<L, TF, C>, dim3(nblk, nout), kThreadsOf<C>, 0, s,
       io, reinterpret_cast<const double2 *>(p.twid), p.coslat, p.P2, p.N + 1, nf, p.jm, resident ? 1 : 0);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
+}
+
+template <int L, typename TF>
+int inv_dispatch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
+  if constexpr (L <= kFftSmallMaxIm)
+    if (fft_block(p, nf) == kFftSmall) return inv_launch<L, kFftSmall>(p, io, nout, nf, s);
+  return inv_launch<L, kFftBig>(p, io, nout, nf, s);
 }
 
 template <typename TF>
@@ -219,18 +235,24 @@ int inv_typed(const Plan &p, const TF *four, const TF *four2, double *grid, doub
 
 }  // namespace
 
-int fft_fwd_capacity(const Plan &p) {
+template <int L>
+void capacities(int device, int (&out)[2]) {
+  out[0] = fwd_resident<L, kFftBig>(device);
+  if constexpr (L <= kFftSmallMaxIm) out[1] = fwd_resident<L, kFftSmall>(device);
+}
+
+void fft_fwd_capacity(const Plan &p, int (&out)[2]) {
+  out[0] = out[1] = 0;
   switch (p.im) {
-    case 16: return fwd_resident<16>(p.device);
-    case 32: return fwd_resident<32>(p.device);
-    case 64: return fwd_resident<64>(p.device);
-    case 128: return fwd_resident<128>(p.device);
-    case 256: return fwd_resident<256>(p.device);
-    case 512: return fwd_resident<512>(p.device);
-    case 1024: return fwd_resident<1024>(p.device);
-    case 2048: return fwd_resident<2048>(p.device);
+    case 16: return capacities<16>(p.device, out);
+    case 32: return capacities<32>(p.device, out);
+    case 64: return capacities<64>(p.device, out);
+    case 128: return capacities<128>(p.device, out);
+    case 256: return capacities<256>(p.device, out);
+    case 512: return capacities<512>(p.device, out);
+    case 1024: return capacities<1024>(p.device, out);
+    case 2048: return capacities<2048>(p.device, out);
   }
-  return 0;
 }
 
 int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s) {

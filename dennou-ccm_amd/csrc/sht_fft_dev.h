@@ -18,15 +18,23 @@
 namespace sht {
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kLdsComplex = 2048;
+// A block transforms C complex values (G = C / L ring pairs) with C / 8
+// threads: kFftBig (256 threads) or kFftSmall (64 threads, one wave), see
+// fft_block.
+template <int C>
+constexpr int kThreadsOf = C / 8;
 // LDS holds one pad element per 16 complex values -- the radix-16 pass writes
 // 16 consecutive outputs per thread (a 256-byte lane stride), which without the
 // pad lands every lane of a ds_write on the same banks -- and one more per
 // transform, because the Fourier-side loops run the G transforms of a block in
 // consecutive lanes (element m of transform g), a stride of L(1 + 1/16)
 // elements that is otherwise a multiple of the bank count.
-constexpr int kLdsPadded = kLdsComplex + kLdsComplex / 16 + kLdsComplex / 16;
+template <int C>
+constexpr int kPaddedOf = C + C / 16 + C / 16;
+// the big block, as the fused inverse uses it
+constexpr int kThreads = kThreadsOf<kFftBig>;
+constexpr int kLdsComplex = kFftBig;
+constexpr int kLdsPadded = kPaddedOf<kFftBig>;
 template <int L>
 __device__ __forceinline__ int px(int i) { return i + (i >> 4) + i / L; }
 
@@ -102,10 +110,11 @@ struct NoHook {
   __device__ void operator()() const {}
 };
 
-template <int L, int R, int Ns, int SIGN, class In, class Out, class Hook = NoHook>
+template <int L, int R, int Ns, int SIGN, int C, class In, class Out, class Hook = NoHook>
 __device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out, int tid, Hook hook = {}) {
 #pragma clang fp contract(off)
-  constexpr int G = kLdsComplex / L;
+  constexpr int G = C / L;
+  constexpr int kThreads = kThreadsOf<C>;
   constexpr int NB = G * (L / R);                       // butterflies in the block
   constexpr int QB = (NB + kThreads - 1) / kThreads;    // per thread
   double2 v[QB][R];
@@ -150,7 +159,7 @@ __device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out,
   __syncthreads();
 }
 
-// Complex FFT of G = kLdsComplex / L transforms, exponent sign SIGN (-1
+// Complex FFT of G = C / L transforms, exponent sign SIGN (-1
 // forward, +1 inverse), unnormalised; tw (in LDS) holds e^{+2 pi i t / L}.
 // The first pass reads through first(g, i) and the last writes through
 // last(g, i, v), so a caller can feed the first butterflies straight from
@@ -159,39 +168,39 @@ __device__ __forceinline__ void stockham_pass(const double2 *tw, In in, Out out,
 // Radix plan: 16 = 16, 32 = 8*4, 64 = 8*8, 128 = 16*8, 256 = 16*16,
 // 512 = 8*8*8, 1024 = 16*16*4, 2048 = 16*16*8.  Ends with a barrier.
 // hook() runs in the first pass once its reads are issued (see stockham_pass).
-template <int L, int SIGN, class First, class Last, class Hook = NoHook>
+template <int L, int SIGN, int C, class First, class Last, class Hook = NoHook>
 __device__ __forceinline__ void fft_lds(double2 *buf, const double2 *tw, First first, Last last, int tid,
                                         Hook hook = {}) {
 #pragma clang fp contract(off)
   auto rd = [&](int g, int i) { return buf[px<L>(g * L + i)]; };
   auto wr = [&](int g, int i, double2 v) { buf[px<L>(g * L + i)] = v; };
   if constexpr (L == 16) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, last, tid, hook);
+    stockham_pass<L, 16, 1, SIGN, C>(tw, first, last, tid, hook);
   } else if constexpr (L == 32) {
-    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 4, 8, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 8, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 4, 8, SIGN, C>(tw, rd, last, tid);
   } else if constexpr (L == 64) {
-    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 8, 8, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 8, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 8, 8, SIGN, C>(tw, rd, last, tid);
   } else if constexpr (L == 128) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 8, 16, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 16, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 8, 16, SIGN, C>(tw, rd, last, tid);
   } else if constexpr (L == 256) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 16, 16, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 16, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 16, 16, SIGN, C>(tw, rd, last, tid);
   } else if constexpr (L == 512) {
-    stockham_pass<L, 8, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 8, 8, SIGN>(tw, rd, wr, tid);
-    stockham_pass<L, 8, 64, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 8, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 8, 8, SIGN, C>(tw, rd, wr, tid);
+    stockham_pass<L, 8, 64, SIGN, C>(tw, rd, last, tid);
   } else if constexpr (L == 1024) {
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 16, 16, SIGN>(tw, rd, wr, tid);
-    stockham_pass<L, 4, 256, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 16, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 16, 16, SIGN, C>(tw, rd, wr, tid);
+    stockham_pass<L, 4, 256, SIGN, C>(tw, rd, last, tid);
   } else {
     static_assert(L == 2048, "unsupported ring length");
-    stockham_pass<L, 16, 1, SIGN>(tw, first, wr, tid, hook);
-    stockham_pass<L, 16, 16, SIGN>(tw, rd, wr, tid);
-    stockham_pass<L, 8, 256, SIGN>(tw, rd, last, tid);
+    stockham_pass<L, 16, 1, SIGN, C>(tw, first, wr, tid, hook);
+    stockham_pass<L, 16, 16, SIGN, C>(tw, rd, wr, tid);
+    stockham_pass<L, 8, 256, SIGN, C>(tw, rd, last, tid);
   }
 }
 
@@ -227,21 +236,22 @@ __device__ __forceinline__ double2 widen(double2 v) { return v; }
 
 // One inverse tile: four (even/odd Legendre sums) -> grid for latitude pair
 // p and field group fgr (tile = fgr * P2 + p).  A thread's Fourier loads (at
-// most 4 (m, field) pairs: M1 * G <= L/2 * G = 1024) and, with tw not NULL,
+// most 4 (m, field) pairs: M1 * G <= L/2 * G = C/2) and, with tw not NULL,
 // the twiddles into twl are issued before any LDS traffic (with tw NULL the
 // caller has filled twl and synchronised).  Only the coefficients m < M1 and
 // L - m are placed in LDS: the first FFT pass takes the band between them as
 // zeros without reading it, and the last pass stores the rings straight to
 // the grid (non-temporal).  mode: 0 value, 1 GradLon (x i m / cos lat),
 // 2 GradLat (x 1 / cos lat).  Ends with a barrier.
-template <int L, typename TF, bool COH>
+template <int L, typename TF, bool COH, int C = kFftBig>
 __device__ __forceinline__ void fft_inv_tile(double2 *buf, double2 *twl, const double2 *__restrict__ tw,
                                              const TF *__restrict__ src, double *__restrict__ dst, int mode,
                                              const double *__restrict__ coslat, int P2, int M1, int nf, int jm,
                                              int tile, int tid) {
 #pragma clang fp contract(off)
-  constexpr int G = kLdsComplex / L;
-  constexpr int NM = (kLdsComplex / 2) / kThreads; // max (m, field) pairs per thread
+  constexpr int G = C / L;
+  constexpr int kThreads = kThreadsOf<C>;
+  constexpr int NM = (C / 2) / kThreads;  // max (m, field) pairs per thread
   const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
   const int nfg = (nf + G - 1) / G;
   const int jn = P2 + p, js = P2 - 1 - p;
@@ -297,7 +307,7 @@ __device__ __forceinline__ void fft_inv_tile(double2 *buf, double2 *twl, const d
       __builtin_nontemporal_store(z.y, dst + ((size_t)f * jm + js) * L + i);
     }
   };
-  fft_lds<L, +1>(buf, twl, spectrum, rings, tid);
+  fft_lds<L, +1, C>(buf, twl, spectrum, rings, tid);
 }
 
 }  // namespace

@@ -56,7 +56,7 @@ int fail(int code, const std::string &msg);
 struct Plan {
   int N = 0, im = 0, jm = 0, P2 = 0, P2p = 0, K = 0, kmax = 0, prec = 0, flags = 0;
   int device = 0;
-  int fft_fwd_resident = 0;  // forward FFT blocks the device holds at once (0: unknown)
+  int fft_fwd_resident[2] = {0, 0};  // forward FFT blocks the device holds at once, big / small (0: unknown)
   int maxke = 0;            // ceil((N+1)/2): most rows of one parity
   // device tables
   double *mu = nullptr, *w = nullptr, *coslat = nullptr; // jm each, S->N
@@ -160,13 +160,29 @@ __device__ __forceinline__ void put_four(T *dst, T v) {
 // launch: the T85 round trip measured 40.15 against 39.40 us (split-BF16
 // x6) and 46.45 against 46.17 us (FP64), so the stores stream.
 __device__ __forceinline__ void put_spec(double *dst, double v) { __builtin_nontemporal_store(v, dst); }
-inline int fft_group(int im) { return 2048 / im; }
-inline FourGeom four_geom(int N, int im, int nf) {
-  const int G = fft_group(im);
+// FFT blocks: a workgroup transforms C complex values, G = C / im ring pairs
+// of one latitude pair, with C / 8 threads.  On grids of at most
+// kFftSmallMaxIm longitudes, calls that give the big blocks fewer than
+// kFftSmallBelow workgroups take the small ones (one wave), which hold the
+// Fourier intermediate in field groups of a quarter the size (fft_block).
+// Measured on MI355X: T21L16 (64 longitudes, 49 fields: 32 big blocks) round
+// trip 15.2 -> 13.9 us; T42 (128) unchanged; T85 (256) with 1-8 fields
+// 0.3-0.6 us slower per FFT launch (a one-wave block hides less latency and
+// still runs whole radix-16 butterflies per thread).
+constexpr int kFftBig = 2048, kFftSmall = 512;
+constexpr int kFftSmallMaxIm = 64, kFftSmallBelow = 256;
+inline int fft_group(int im) { return kFftBig / im; }  // the largest field group (allocation)
+inline int fft_block(const Plan &p, int nf) {
+  const int g = kFftBig / p.im;
+  return p.im <= kFftSmallMaxIm && (long)p.P2 * ((nf + g - 1) / g) < kFftSmallBelow ? kFftSmall : kFftBig;
+}
+inline FourGeom four_geom_g(int N, int G, int nf) {
   int lg = 0;
   while ((1 << lg) < 2 * G) ++lg;
   return FourGeom{N + 1, (nf + G - 1) / G, lg};
 }
+// the layout a call with nf fields uses (its FFT block's field groups)
+inline FourGeom four_geom(const Plan &p, int nf) { return four_geom_g(p.N, fft_block(p, nf) / p.im, nf); }
 // Geometry for the Legendre inverse, which writes the Fourier intermediate:
 // non-temporal stores up to kStreamFourBytes of it, regular stores above.
 // Measured on MI355X at T85 (97 to 388 fields): streaming saves the write-back
@@ -174,8 +190,8 @@ inline FourGeom four_geom(int N, int im, int nf) {
 // ~25 MB the following FFT reads the regularly stored intermediate from the
 // Infinity Cache 6 us faster than the streamed one.
 constexpr size_t kStreamFourBytes = (size_t)24 << 20;
-inline FourGeom four_geom_out(const Plan &p, int nf, size_t elem_bytes) {
-  FourGeom g = four_geom(p.N, p.im, nf);
+inline FourGeom four_geom_out(const Plan &p, int nf, size_t elem_bytes, int G = 0) {
+  FourGeom g = G ? four_geom_g(p.N, G, nf) : four_geom(p, nf);
   const size_t bytes = (size_t)p.P2 * g.nfg * g.M1 * 2 * ((size_t)1 << g.lg) * elem_bytes;
   g.stream = bytes <= kStreamFourBytes;
   return g;
@@ -218,7 +234,7 @@ int launch_gauss(int jm, double *mu, double *w, double *coslat, hipStream_t s);
 // FFT stage (sht_fft.hip).  mode: 0 value, 1 GradLon (x i m / cos), 2 GradLat (x 1/cos).
 // f32 = the Fourier intermediate is FP32 (mixed-precision plans); the FFT itself is FP64.
 // Forward FFT blocks one device holds at once (occupancy x CUs; 0 if unknown).
-int fft_fwd_capacity(const Plan &p);
+void fft_fwd_capacity(const Plan &p, int (&out)[2]);
 int launch_fft_fwd(const Plan &p, const double *grid, void *four, bool f32, int nf, hipStream_t s);
 // Divergence analysis FFT: a -> fa with x i m / cos, b -> fb with x 1 / cos (FP64;
 // either pair may be NULL), one launch.

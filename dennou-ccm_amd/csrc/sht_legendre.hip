@@ -639,7 +639,7 @@ int fwd_tile_launch(const Plan &p, KSplit ks, const double *four, double *spec, 
   const size_t lds = ks.m2 > 0 ? sizeof(double) * 4 * MAXRT * 256 : 0;
   launch_kernel(leg_fwd_f64_tile_kernel<MAXRT, D>, (int)nblk, 256, lds, s,
       four, spec, p.Pfwd, p.poff_fwd, p.zero, p.N, p.P2, p.P2p, ncol, nct, ks, p.xcd_m[1],
-      p.xcd_mlen[1], four_geom(p.N, p.im, nf));
+      p.xcd_mlen[1], four_geom(p, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -650,7 +650,7 @@ int fwd_wave_launch(const Plan &p, const double *four, double *spec, int nf, hip
   const int nunits = p.n_fwd_units * nct;
   launch_kernel(leg_fwd_f64_wave_kernel<D>, (nunits + 3) / 4, 256, 0, s,
       four, spec, p.Pfwd, p.poff_fwd, p.fwd_units, p.N, p.P2, p.P2p, ncol, nct, nunits,
-      four_geom(p.N, p.im, nf));
+      four_geom(p, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -835,7 +835,7 @@ int fwd_staged_launch(const Plan &p, const double *four, double *spec, int nf, h
   const size_t lds = sizeof(double) * CT * 2 * p.P2p * 16;  // <= 64 KB: P2p <= 256
   launch_kernel(leg_fwd_f64_staged_kernel<CT, 4, NW>, xcd_m_blocks(p, 1, (nct + CT - 1) / CT), NW * 64, lds,
       s, four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
-      four_geom(p.N, p.im, nf));
+      four_geom(p, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -999,7 +999,7 @@ int fwd_halves_launch(const Plan &p, const double *four, double *spec, int nf, h
   const size_t lds = sizeof(double) * CT * 2 * (p.P2p / 2) * 16;
   launch_kernel(leg_fwd_f64_halves_kernel<CT, 4, 8, MAXU>, xcd_m_blocks(p, 1, (nct + CT - 1) / CT), 512, lds,
       s, four, spec, p.Pfwd, p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
-      four_geom(p.N, p.im, nf));
+      four_geom(p, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -1040,7 +1040,7 @@ int fwd_launch(const Plan &p, const double *four, double *spec, int nf, hipStrea
   const size_t lds = sizeof(double) * kWaves * MAXRT * 256;
   launch_kernel(leg_fwd_f64_kernel<MAXRT>, nblk, 256, lds, s, four, spec, table ? table : p.Pfwd,
                 p.poff_fwd, p.xcd_m[1], p.xcd_mlen[1], p.N, p.P2, p.P2p, ncol, nct,
-                four_geom(p.N, p.im, nf), swap ? 1 : 0, accum ? 1 : 0);
+                four_geom(p, nf), swap ? 1 : 0, accum ? 1 : 0);
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }

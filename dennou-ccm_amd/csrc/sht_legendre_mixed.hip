@@ -1076,7 +1076,7 @@ int fwd_nsl(const Plan &p, int shape, const float *four, double *spec, int nf, h
     launch_kernel(leg_fwd_mixed_wave_kernel<MODE, NSL>, xcd_m_blocks(p, 1, (nct + 3) / 4), 256, 0, s,
                   four, spec, static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd,
                   p.xcd_m[1], p.xcd_mlen[1], p.zero, p.N, p.P2, roundup(p.P2p, 32), ncol, nct,
-                  four_geom(p.N, p.im, nf));
+                  four_geom(p, nf));
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }
@@ -1090,7 +1090,7 @@ int fwd_nsl(const Plan &p, int shape, const float *four, double *spec, int nf, h
   const int ct = shape == SHT_SHAPE_TILE ? 2 : 1;
   launch_kernel(kern, xcd_m_blocks(p, 1, (nct + ct - 1) / ct), eight ? 512 : 256, 0, s, four, spec,
                 static_cast<const uint16_t *>(p.Lfwd), p.loff_fwd, p.lexp_fwd, p.xcd_m[1],
-                p.xcd_mlen[1], p.N, p.P2, roundup(p.P2p, 32), ncol, nct, four_geom(p.N, p.im, nf));
+                p.xcd_mlen[1], p.N, p.P2, roundup(p.P2p, 32), ncol, nct, four_geom(p, nf));
   SHT_HIP(hipGetLastError());
   return SHT_OK;
 }
@@ -1170,7 +1170,7 @@ int launch_inv_fused(const Plan &p, const double *spec, float *four, double *gri
   a.nct = (a.ncol + 15) / 16;
   a.nf = nf;
   a.jm = p.jm;
-  a.geo = four_geom_out(p, nf, sizeof(float));
+  a.geo = four_geom_out(p, nf, sizeof(float), fft_group(p.im));  // the big FFT blocks
   // no more workgroups than items, a multiple of 8 (one queue per XCD)
   const long items = 8L * a.mlen * a.nct + (long)p.P2 * a.geo.nfg;
   const int nblk = (int)std::max(8L, std::min<long>(p.fused_blocks, items) / 8 * 8);

@@ -58,7 +58,9 @@ int sht_spec_to_grid_dev_noalloc(sht_plan *h, const double *spec, double *grid, 
   if (nf < 1 || nf > p.kmax)
     return fail(SHT_ERR_INVALID, "sht_spec_to_grid_dev_noalloc: nfield must be in [1, kmax]");
   // the intermediate [P2][ceil(nf/G)][N+1][2][2G] in FP64, FP32 in the mixed forms
-  const FourGeom g = four_geom(p.N, p.im, nf);
+  // the largest layout any shape of the call can use (the fused inverse
+  // always takes the big FFT blocks)
+  const FourGeom g = four_geom_g(p.N, fft_group(p.im), nf);
   const size_t elem = p.prec == SHT_PREC_FP64 ? sizeof(double) : sizeof(float);
   const size_t need = (size_t)p.P2 * g.nfg * g.M1 * 2 * ((size_t)1 << g.lg) * elem;
   if (need > kScratchBytes)

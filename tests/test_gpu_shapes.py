@@ -296,3 +296,28 @@ def test_split_classes_meet_the_bar(N, im, jm, nf):
         p.set_classes(nf, shtamd.DIR_FORWARD, 0, M + 1)
     p.set_classes(nf, shtamd.DIR_INVERSE, -1, -1)  # back to the built-in classes
     p.close()
+
+
+@pytest.mark.parametrize("prec", [shtamd.PREC_FP64, shtamd.PREC_SPLIT_BF16X6])
+@pytest.mark.parametrize("N,im,jm,small,big", [(21, 64, 32, 49, 600), (7, 32, 16, 5, 2100), (5, 16, 12, 3, 5600)])
+def test_fft_block_size_does_not_change_results(prec, N, im, jm, small, big):
+    """On grids of at most 64 longitudes, calls that give the FFT few
+    workgroups run it in small blocks (512 complex values, one wave), the
+    others in big ones (2048, four waves), each with its own field-group
+    layout of the Fourier intermediate; every transform and every Legendre
+    sum is the same computation, so a field's grid and spectra are bitwise
+    the same in a small-block call and a big-block call."""
+    say("fft blocks", prec, N, small, big)
+    spec = synth.spectra(N, big, cfg=17)
+    p = shtamd.Plan(N, im, jm, big, precision=prec)
+    p.set_shapes(0, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT)
+    g_big = run_inverse(p, spec, big)[0]
+    g_small = run_inverse(p, spec[:, :small], small)[0]
+    assert np.array_equal(g_big[:small], g_small)
+    s_big = run_forward(p, g_big, big)
+    s_small = run_forward(p, g_big[:small], small)
+    assert np.array_equal(s_big[:, :small], s_small)
+    ref = oracle.Plan(N, im, jm)
+    tol = TOL if prec == shtamd.PREC_FP64 else 1e-6
+    assert rel(g_small, ref.spec_to_grid(spec[:, :small])) < tol
+    p.close()
