@@ -46,18 +46,22 @@ static int check_call(const sht_plan *h, const void *a, const void *b, int nf, c
   return SHT_OK;
 }
 
+// Created on the first host call; each part only while missing, so a call
+// after a failed set-up completes it rather than running without it.
 static int ensure_host_staging(Plan &p) {
-  if (p.pinned) return SHT_OK;
   const size_t grid = (size_t)p.kmax * p.jm * p.im;
   const size_t wa = (size_t)p.kmax * (p.N + 1) * (p.N + 1);
   const size_t spec = (size_t)p.kmax * p.K * 2;
   const size_t big = grid > wa ? grid : wa;
-  SHT_HIP(hipStreamCreateWithFlags(&p.host_stream, hipStreamNonBlocking));
-  SHT_HIP(hipMalloc(&p.stage_a, sizeof(double) * big));
-  SHT_HIP(hipMalloc(&p.stage_b, sizeof(double) * spec));
-  SHT_HIP(hipHostMalloc(&p.pinned, sizeof(double) * big, hipHostMallocDefault));
-  p.pinned_bytes = sizeof(double) * big;
-  for (hipEvent_t &e : p.host_events) SHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!p.host_stream) SHT_HIP(hipStreamCreateWithFlags(&p.host_stream, hipStreamNonBlocking));
+  if (!p.stage_a) SHT_HIP(hipMalloc(&p.stage_a, sizeof(double) * big));
+  if (!p.stage_b) SHT_HIP(hipMalloc(&p.stage_b, sizeof(double) * spec));
+  for (hipEvent_t &e : p.host_events)
+    if (!e) SHT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!p.pinned) {
+    SHT_HIP(hipHostMalloc(&p.pinned, sizeof(double) * big, hipHostMallocDefault));
+    p.pinned_bytes = sizeof(double) * big;
+  }
   return SHT_OK;
 }
 
