@@ -267,32 +267,33 @@ def run_host_dropin(precision: int, reps: int = 20):
             "note": "host arrays, PCIe copies included; not the bench value"}
 
 
-SMALL_CONFIGS = {  # the other single-GPU configurations: (N, im, jm, levels, gradients)
-    0: (21, 64, 32, 16, False),   # T21L16 round trip
-    1: (42, 128, 64, 26, True),   # T42L26: inverse with GradLon/GradLat, forward of a temperature grid
+SMALL_WORKLOADS = {  # (N, im, jm, levels, gradients)
+    "t21l16": (21, 64, 32, 16, False),  # round trip
+    "t42l26": (42, 128, 64, 26, True),  # inverse with GradLon/GradLat, forward of a temperature grid
 }
 
 
-def run_small_config(cfg: int, steps: int, cpu_seconds: float):
-    """Configurations [0] and [1]: GPU FP64 and split-BF16 x6 steps (eager,
-    queue prefilled, events on the stream) and the CPU oracle on the host's
-    cores; errors against the FP64 oracle.  Returns a JSON-ready dict."""
+def run_small_workload(name: str, steps: int, cpu_seconds: float, seed_cfg: int):
+    """The T21L16 round trip and the T42L26 step with gradients: GPU FP64 and
+    split-BF16 x6 steps (eager, queue prefilled, events on the stream) and
+    the CPU oracle on the host's cores; errors against the FP64 oracle.
+    Returns a JSON-ready dict."""
     import torch
 
     import oracle
     import shtamd
     from shtamd import synth
 
-    n, im, jm, nlev, grads = SMALL_CONFIGS[cfg]
+    n, im, jm, nlev, grads = SMALL_WORKLOADS[name]
     nf = 3 * nlev + 1
-    spec_c = synth.state_spectra(n, nlev, cfg=cfg)
+    spec_c = synth.state_spectra(n, nlev, cfg=seed_cfg)
     ref = oracle.Plan(n, im, jm)
-    out = {"workload": f"config [{cfg}]: T{n}L{nlev}, {nf} fields, {im}x{jm} Gaussian grid"
+    out = {"workload": f"T{n}L{nlev}: {nf} fields, {im}x{jm} Gaussian grid"
                        + (", inverse with GradLon/GradLat + forward of a 200-300 K grid" if grads else
                           ", inverse + forward"), "forms": {}}
     # the temperature grid on the oracle's Gauss latitudes (the GPU plan's own
-    # agree to ~1e-16); forward of it is part of config [1]'s step
-    temp = synth.temperature_grid(oracle.gauss(jm)[2], im, nf, cfg=cfg) if grads else None
+    # agree to ~1e-16); its forward transform is part of the step
+    temp = synth.temperature_grid(oracle.gauss(jm)[2], im, nf, cfg=seed_cfg) if grads else None
     for form in ("fp64", "bf16x6"):
         plan = shtamd.Plan(n, im, jm, nf, precision=PRECISIONS[form],
                            flags=shtamd.FLAG_GRAD if grads else 0)
@@ -500,7 +501,8 @@ def main():
     ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve", "copycal", "small"],
                     help="ensemble: the 64-member mixed-truncation batch; serve: request streams at "
                          "100-1000 req/s (separate JSON lines); copycal: a plain 64 MiB device copy, "
-                         "three dispatches, the calibration run for rocprofv3's FETCH_SIZE/WRITE_SIZE")
+                         "three dispatches, the calibration run for rocprofv3's FETCH_SIZE/WRITE_SIZE; "
+                         "small: the T21L16 round trip and the T42L26 step with gradients, GPU and CPU oracle")
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests in the 500 req/s serving run of the headline line (0 = skip)")
     ap.add_argument("--members", type=int, default=64)
@@ -518,8 +520,8 @@ def main():
         print(json.dumps(copy_calibration()))
         return
     if args.workload == "small":
-        for cfg in SMALL_CONFIGS:
-            print(json.dumps(run_small_config(cfg, args.steps, min(args.cpu_seconds, 5.0))), flush=True)
+        for k, name in enumerate(SMALL_WORKLOADS):
+            print(json.dumps(run_small_workload(name, args.steps, min(args.cpu_seconds, 5.0), k)), flush=True)
         return
 
     import oracle
