@@ -251,6 +251,7 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   p.flags = desc->flags;
   p.maxke = ceil_half(p.N + 1);
   p.alloc_fail = host::injected_count("SHT_INJECT_PLAN_FAILURE");
+  p.fused_inject = host::injected_count("SHT_INJECT_FUSED_TIMEOUT") > 0;
   hipError_t e = hipGetDevice(&p.device);
   if (e != hipSuccess) {
     delete h;
@@ -277,6 +278,11 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   }
   *out = h;
   return SHT_OK;
+}
+
+int sht_plan_fused_failures(const sht_plan *h, long long *count) {
+  if (!h || !count) return fail(SHT_ERR_INVALID, "sht_plan_fused_failures: NULL argument");
+  return fused_wait_failures(h->p, count);
 }
 
 void sht_plan_destroy(sht_plan *h) {

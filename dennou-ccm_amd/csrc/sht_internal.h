@@ -86,6 +86,7 @@ struct Plan {
   // (0: not available)
   int *fsync = nullptr;
   int fused_blocks = 0;
+  int fused_inject = 0;  // SHT_INJECT_FUSED_TIMEOUT: every fused wait times out
   // Legendre kernel shapes (SHT_SHAPE_*): the default for every field count,
   // and entries fixed by sht_plan_set_shapes / sht_plan_autotune for one count
   // and the wavenumber classes of the split shape (host::Tuning: K loop split
@@ -279,6 +280,8 @@ int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf,
 constexpr int kFusedSyncHead = 18, kSyncStride = 64;
 bool fused_inv_available(const Plan &p);
 int launch_inv_fused(const Plan &p, const double *spec, float *four, double *grid, int nf, hipStream_t s);
+// Waits of the plan's fused calls that gave up (synchronous read; sticky).
+int fused_wait_failures(const Plan &p, long long *count);
 int launch_leg_inv_mixed_shape(const Plan &p, int shape, const double *spec, float *four, int nf,
                                hipStream_t s);
 int launch_leg_fwd_mixed_shape(const Plan &p, int shape, const float *four, double *spec, int nf,

@@ -487,6 +487,8 @@ struct FusedInv {
   const double *coslat;
   int *sync;
   int mlen, N, P2, P2p, ncol, nct, nf, jm;
+  int need;             // completions a column tile waits for: N + 1
+  uint64_t spin_ticks;  // bound of one wait (s_memrealtime ticks)
   FourGeom geo;
 };
 
@@ -496,6 +498,10 @@ struct FusedInv {
 // workgroups, 17 errors, kFusedSyncHead + c column tile c's count.
 constexpr int kSyncL = 0, kSyncF = 8, kSyncDone = 16, kSyncErr = 17;
 constexpr uint64_t kFusedSpinTicks = 100000000;  // s_memrealtime runs at 100 MHz: one second
+// SHT_INJECT_FUSED_TIMEOUT: every wait of the plan's fused calls gives up
+// after 10 us (the completion count asked for is one more than the items
+// make), which exercises the timeout path: NaN rings and the error count
+constexpr uint64_t kFusedInjectedSpinTicks = 1000;
 
 __device__ __forceinline__ int sync_load(const int *a) {
   return __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -519,7 +525,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void i
   auto slot = [&](int k) { return sy + k * kSyncStride; };
   const int nL = a.mlen * a.nct;                // Legendre items per queue
   const int nF = a.P2 * a.geo.nfg, nFx = (nF - x + 7) >> 3;
-  const int need = a.N + 1;
+  const int need = a.need;
   for (int q = tid; q < L; q += 256) twl[q] = a.tw[q];
   // Next item, chosen by thread 0: >= 0 Legendre item (queue << 24 | index),
   // <= -2 FFT tile -2 - tile, kDone.  Thread 0 keeps its queue state.
@@ -581,7 +587,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void i
           const int it = steal();
           if (it >= 0) return it;
         }
-        if (__builtin_amdgcn_s_memrealtime() - t0 > kFusedSpinTicks) {
+        if (__builtin_amdgcn_s_memrealtime() - t0 > a.spin_ticks) {
           sync_add(slot(kSyncErr), 1);
           bad = 1;
           ready = true;
@@ -1149,6 +1155,15 @@ int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf,
 
 bool fused_inv_available(const Plan &p) { return fused_supported(p) && p.fsync && p.fused_blocks >= 8; }
 
+int fused_wait_failures(const Plan &p, long long *count) {
+  *count = 0;
+  if (!p.fsync) return SHT_OK;
+  int v = 0;
+  SHT_HIP(hipMemcpy(&v, p.fsync + kSyncErr * kSyncStride, sizeof(int), hipMemcpyDeviceToHost));
+  *count = v;
+  return SHT_OK;
+}
+
 int launch_inv_fused(const Plan &p, const double *spec, float *four, double *grid, int nf, hipStream_t s) {
   if (!fused_inv_available(p)) return fail(SHT_ERR_UNSUPPORTED, "fused inverse: not available for this plan");
   FusedInv a{};
@@ -1171,6 +1186,8 @@ int launch_inv_fused(const Plan &p, const double *spec, float *four, double *gri
   a.nf = nf;
   a.jm = p.jm;
   a.geo = four_geom_out(p, nf, sizeof(float), fft_group(p.im));  // the big FFT blocks
+  a.need = p.N + 1 + (p.fused_inject ? 1 : 0);
+  a.spin_ticks = p.fused_inject ? kFusedInjectedSpinTicks : kFusedSpinTicks;
   // no more workgroups than items, a multiple of 8 (one queue per XCD)
   const long items = 8L * a.mlen * a.nct + (long)p.P2 * a.geo.nfg;
   const int nblk = (int)std::max(8L, std::min<long>(p.fused_blocks, items) / 8 * 8);

@@ -123,6 +123,7 @@ SIGNATURES = {
     "sht_remap_apply_dev": (_ci, [_vp, _vp, _vp, _ci, _vp]),
     "sht_plan_set_shapes": (_ci, [_vp, _ci, _ci, _ci]),
     "sht_plan_get_shapes": (_ci, [_vp, _ci, ctypes.POINTER(_ci), ctypes.POINTER(_ci)]),
+    "sht_plan_fused_failures": (_ci, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "sht_plan_autotune": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _dp]),
     "sht_plan_autotune_flags": (_ci, [_vp, _ci, _ci, ctypes.c_char_p, _ci, _dp]),
     "sht_batch_autotune_flags": (_ci, [_vp, _ci, ctypes.c_char_p, _ci]),
@@ -306,6 +307,12 @@ class Plan:
         a, b = ctypes.c_int(), ctypes.c_int()
         check(lib().sht_plan_get_shapes(self._h, nfield, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def fused_failures(self) -> int:
+        """Waits of fused-shape calls on this plan that gave up (see sht.h)."""
+        n = ctypes.c_longlong(0)
+        check(lib().sht_plan_fused_failures(self._h, ctypes.byref(n)))
+        return n.value
 
     def set_classes(self, nfield: int, direction: int, m4: int, m2: int):
         """Wavenumber classes of the split shape for calls with nfield fields."""

@@ -364,6 +364,13 @@ enum {
 int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
 /* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
 int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);
+/* Waits of SHT_SHAPE_FUSED calls on the plan that gave up: a fused call's FFT
+ * tile waits at most one second for its column tile's Legendre items (a
+ * broken device or counter state), then sets its grid rows to NaN and counts
+ * here.  Sticky for the plan's life; reads the device synchronously.  No
+ * SPML counterpart.  SHT_INJECT_FUSED_TIMEOUT=1 in the environment at plan
+ * creation makes every such wait give up after 10 us (failure testing). */
+int sht_plan_fused_failures(const sht_plan *plan, long long *count);
 /* Wavenumber classes of SHT_SHAPE_TILE_SPLIT for calls with nf fields in one
  * direction (SHT_DIR_INVERSE or SHT_DIR_FORWARD): wavenumbers are numbered
  * heaviest first (m = 0 has the longest K loop), the loop runs split four

@@ -166,3 +166,33 @@ def test_fused_in_autotuner_and_gradients():
     for kind, o in enumerate(outs):
         assert rel(o.cpu().numpy(), ref.spec_to_grid(spec, kind=kind)) < 1e-12, kind
     p.close()
+
+
+def test_fused_wait_timeout_is_reported(monkeypatch):
+    """A fused call whose column tiles never complete (forced from the host:
+    SHT_INJECT_FUSED_TIMEOUT asks for one completion more than the items make
+    and bounds each wait at 10 us) must still end: every FFT tile gives up,
+    sets its grid rows to NaN and counts the failure.  The counters are left
+    clean for the next call, and a plan created without the injection is
+    unaffected."""
+    N, im, jm, nf = 42, 128, 64, 26
+    prec = shtamd.PREC_SPLIT_BF16X6
+    spec = synth.spectra(N, nf, cfg=80)
+    monkeypatch.setenv("SHT_INJECT_FUSED_TIMEOUT", "1")
+    p = shtamd.Plan(N, im, jm, nf, precision=prec)
+    monkeypatch.delenv("SHT_INJECT_FUSED_TIMEOUT")
+    p.set_shapes(0, shtamd.SHAPE_FUSED, shtamd.SHAPE_KSPLIT)
+    assert p.fused_failures() == 0
+    for call in range(2):
+        say("forced timeout", call)
+        g = inverse(p, spec, nf)
+        assert np.isnan(g).all(), call
+        # one give-up per FFT tile: latitude pairs x field groups of 16 fields
+        assert p.fused_failures() == (call + 1) * (jm // 2) * ((nf + 15) // 16)
+    q = shtamd.Plan(N, im, jm, nf, precision=prec)
+    q.set_shapes(0, shtamd.SHAPE_FUSED, shtamd.SHAPE_KSPLIT)
+    g = inverse(q, spec, nf)
+    assert q.fused_failures() == 0
+    assert rel(g, oracle.Plan(N, im, jm).spec_to_grid(spec)) < TOL[prec]
+    p.close()
+    q.close()
