@@ -274,6 +274,9 @@ int launch_spec_lapla(const Plan &p, const double *in, double *out, int power, i
 // Legendre stage, mixed precision (sht_legendre_mixed.hip)
 int build_mixed_tables(Plan &p);
 int launch_leg_inv_mixed(const Plan &p, const double *spec, float *four, int nf, hipStream_t s);
+// Streaming inverse of the mixed forms (SHT_SHAPE_STAGED): eight waves hold
+// the table of up to 8 row tiles, so grids of at most 128 latitude pairs.
+inline bool mixed_stream_available(const Plan &p) { return p.prec != SHT_PREC_FP64 && p.P2p <= 128; }
 // Fused inverse of the mixed forms (SHT_SHAPE_FUSED): Legendre step and FFT
 // in one persistent launch, grid values only; available for im in [64, 512]
 // and nmax < 192.

@@ -1083,8 +1083,16 @@ int resolve_shape(const Plan &p, int nf, int dir) {
   if (d != SHT_SHAPE_AUTO) return d;
   // mixed forms: the shapes give bitwise equal results, so the choice is by
   // speed alone -- two column tiles per workgroup from 128 latitude pairs
-  // (T170: faster in both directions at 97 to 776 fields), one below
-  if (p.prec != SHT_PREC_FP64) return p.P2p >= 128 ? SHT_SHAPE_TILE : SHT_SHAPE_KSPLIT;
+  // (T170: faster in both directions at 97 to 776 fields), one below; the
+  // six-product inverse streams from 300 fields on that grid (T170 inverse
+  // Legendre step, TILE against STAGED: 200 fields 49.0 / 52.8 us, 388
+  // fields 95.8 / 90.3, 776 fields 235 / 211, 3190 fields 872 / 792; the
+  // two-piece forms stay faster with TILE, 184 / 191 us at 776 fields)
+  if (p.prec != SHT_PREC_FP64) {
+    if (dir == 0 && p.prec == SHT_PREC_SPLIT_BF16X6 && nf >= 300 && mixed_stream_available(p) && p.P2p >= 128)
+      return SHT_SHAPE_STAGED;
+    return p.P2p >= 128 ? SHT_SHAPE_TILE : SHT_SHAPE_KSPLIT;
+  }
   // measured on MI355X from T21 to T170 and 16 to 388 fields: the staged
   // shape wins the forward nearly everywhere and the inverse from 32 fields;
   // below that a wavefront per tile fills the chip better.  STAGED and WAVE

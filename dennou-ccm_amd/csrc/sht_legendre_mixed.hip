@@ -317,6 +317,55 @@ __device__ __forceinline__ void put_four1(float *dst, float v) {
     put_four<NT>(dst, v);
 }
 
+// One row tile rt (16 latitude pairs, its table fragments pa) against one
+// staged column tile jt (groups par * NSL + s at bt, FP16 exponents at
+// ebits): the nsl K slices' products summed in FP64 per parity, then the FP32
+// sums to the Fourier intermediate.  ob is the wave's LDS patch for the
+// output transpose.
+template <int MODE, int NSL, bool NT, bool COH>
+__device__ __forceinline__ void inv_tile(const uint16_t *bt, const int *ebits, float *ob,
+                                         const uint4 (&pa)[2][NSL][3], double ea0, double ea1, int nsl,
+                                         float *__restrict__ four, FourGeom geo, int P2, int ncol, int m,
+                                         int rt, int jt, int lane) {
+  const int r16 = lane & 15, kq = lane >> 4, col = jt * 16 + r16;
+  d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+  for (int s = 0; s < NSL; ++s) {
+    if (s >= nsl) break;
+    uint4 pb0[3], pb1[3];
+    load_pb<MODE>(bt, s, kq, r16, pb0);
+    load_pb<MODE>(bt, NSL + s, kq, r16, pb1);
+    const double sc0 = Form<MODE>::kScaled ? ea0 * ldexp(1.0, -ebits[s * 16 + r16]) : 1.0;
+    const double sc1 = Form<MODE>::kScaled ? ea1 * ldexp(1.0, -ebits[(NSL + s) * 16 + r16]) : 1.0;
+    slice_mac2<MODE>(pa[0][s], pb0, sc0, acc[0], pa[1][s], pb1, sc1, acc[1]);
+  }
+  if (geo.lg >= 2) {
+    // 4 consecutive columns per lane: through the wave's LDS patch
+    // [16 pairs][2 parities][16 columns] (row pitch kOutPitch); the
+    // padding columns of the last field group are written too (zeros)
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) ob[(kq * 4 + i) * kOutPitch + par * 16 + r16] = (float)acc[par][i];
+    const int cend = geo.nfg << geo.lg;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = lane + 64 * h, pl = q >> 3, par = (q >> 2) & 1, c4 = (q & 3) * 4;
+      const int p = rt * 16 + pl, cc = jt * 16 + c4;
+      const f4 v = *reinterpret_cast<const f4 *>(ob + pl * kOutPitch + par * 16 + c4);
+      if (p < P2 && cc < cend) put_four4<NT, COH>(four + geo.at(p, m, par, cc), v);
+    }
+  } else if (col < ncol) {  // field groups of fewer than 4 columns (im = 2048)
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int p = rt * 16 + kq * 4 + i;
+        if (p < P2) put_four1<NT, COH>(four + geo.at(p, m, par, col), (float)acc[par][i]);
+      }
+  }
+}
+
 // LDS of one inverse workgroup item (bytes, 16-byte aligned parts): the split
 // data tiles, their FP16 exponents (scaled form only), the per-wave output
 // transpose.
@@ -396,45 +445,9 @@ __device__ __forceinline__ void leg_inv_mixed_item(
 #pragma unroll
     for (int j = 0; j < CT; ++j) {
       if (j >= nc) break;
-      const int g0 = j * NG, col = (ct0 + j) * 16 + r16;
-      d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
-#pragma unroll
-      for (int s = 0; s < NSL; ++s) {
-        if (s >= nsl) break;
-        uint4 pb0[3], pb1[3];
-        load_pb<MODE>(bt, g0 + s, kq, r16, pb0);
-        load_pb<MODE>(bt, g0 + NSL + s, kq, r16, pb1);
-        const double sc0 = Form<MODE>::kScaled ? ea0 * ldexp(1.0, -ebits[(g0 + s) * 16 + r16]) : 1.0;
-        const double sc1 =
-            Form<MODE>::kScaled ? ea1 * ldexp(1.0, -ebits[(g0 + NSL + s) * 16 + r16]) : 1.0;
-        slice_mac2<MODE>(pa[0][s], pb0, sc0, acc[0], pa[1][s], pb1, sc1, acc[1]);
-      }
-      if (geo.lg >= 2) {
-        // 4 consecutive columns per lane: through this wave's LDS patch
-        // [16 pairs][2 parities][16 columns] (row pitch kOutPitch); the
-        // padding columns of the last field group are written too (zeros)
-        float *ob = obuf + wave * (16 * kOutPitch);
-#pragma unroll
-        for (int par = 0; par < 2; ++par)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) ob[(kq * 4 + i) * kOutPitch + par * 16 + r16] = (float)acc[par][i];
-        const int cend = geo.nfg << geo.lg;
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int q = lane + 64 * h, pl = q >> 3, par = (q >> 2) & 1, c4 = (q & 3) * 4;
-          const int p = rt * 16 + pl, cc = (ct0 + j) * 16 + c4;
-          const f4 v = *reinterpret_cast<const f4 *>(ob + pl * kOutPitch + par * 16 + c4);
-          if (p < P2 && cc < cend) put_four4<NT, COH>(four + geo.at(p, m, par, cc), v);
-        }
-      } else if (col < ncol) {  // field groups of fewer than 4 columns (im = 2048)
-#pragma unroll
-        for (int par = 0; par < 2; ++par)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int p = rt * 16 + kq * 4 + i;
-            if (p < P2) put_four1<NT, COH>(four + geo.at(p, m, par, col), (float)acc[par][i]);
-          }
-      }
+      inv_tile<MODE, NSL, NT, COH>(bt + (size_t)j * NG * NP * 16 * kRowPitch, ebits + j * NG * 16,
+                                   obuf + wave * (16 * kOutPitch), pa, ea0, ea1, nsl, four, geo, P2, ncol,
+                                   m, rt, ct0 + j, lane);
     }
   }
 }
@@ -827,6 +840,109 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
   }
 }
 
+// Streaming inverse (shape STAGED of the mixed forms, grids of 128 latitude
+// pairs): the table stays in registers while the data stream past it.
+// Workgroup = (m, a chunk of cs column tiles) of eight waves; wave w holds the
+// table fragments of row tile w for the whole chunk, so each m's table is
+// read once per chunk instead of once per one or two column tiles.  The
+// spectral tiles pass through two LDS buffers: wave w stages group w of each
+// column tile (2 nsl <= 8 groups: parity, K slice) from registers that are
+// loaded PD tiles ahead, so while the waves multiply column tile j the loads
+// of tiles j + 1 .. j + PD are in flight.  Per output tile the products,
+// their order and the stores are those of leg_inv_mixed_item, so the results
+// equal the other shapes' bitwise.  One workgroup per CU (the table takes
+// 72 of the six-product form's ~210 registers at 128 pairs).
+template <int MODE, int NSL, bool NT, int PD>
+__global__ __launch_bounds__(512) void leg_inv_mixed_stream_kernel(
+    const double *__restrict__ spec, float *__restrict__ four, const uint16_t *__restrict__ L,
+    const int *__restrict__ loff, const int *__restrict__ lexp, const int *__restrict__ mlist,
+    int mlen, int N, int P2, int P2p, int ncol, int nct, int cs, FourGeom geo) {
+  constexpr int NW = 8, NP = Form<MODE>::NP;
+  using One = InvItemLds<MODE, NSL, 1, NW>;
+  constexpr size_t kBuf = One::bt + One::ebits;
+  __shared__ __align__(16) unsigned char lds[2 * kBuf + One::obuf];
+  int m, t;
+  if (!xcd_m_block(blockIdx.x, mlist, mlen, (nct + cs - 1) / cs, m, t)) return;
+  const int wave = wave_id(), lane = threadIdx.x & 63, c = lane & 15, k8 = lane >> 4;
+  const int j0 = t * cs, nj = nct - j0 < cs ? nct - j0 : cs;
+  const int nn = N - m + 1, ke = (nn + 1) >> 1, ko = nn >> 1;
+  const int KP32 = (ke + 31) & ~31, nsl = KP32 >> 5;
+  const double *__restrict__ sm = spec + (size_t)coef_off(N, m) * ncol;
+  const uint16_t *__restrict__ Lm = L + loff[m];
+  const bool mma = wave < (P2p >> 4);
+  uint4 pa[2][NSL][3];
+  if (mma) {
+#pragma unroll
+    for (int par = 0; par < 2; ++par)
+#pragma unroll
+      for (int s = 0; s < NSL; ++s)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          pa[par][s][q] = (q < NP && s < nsl) ? *reinterpret_cast<const uint4 *>(
+                                                    Lm + ((size_t)(par * NP + q) * P2p + wave * 16 + c) * KP32 +
+                                                    s * 32 + k8 * 8)
+                                              : make_uint4(0, 0, 0, 0);
+  }
+  // this wave's group: parity gpar, K slice gs, LDS slot gpar NSL + gs
+  const bool stg = wave < 2 * nsl;
+  const int gpar = wave >= nsl, gs = wave - gpar * nsl, kend = gpar ? ko : ke;
+  auto load = [&](int j, double (&v)[8]) {
+    const int col = (j0 + j) * 16 + c;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int kk = gs * 32 + k8 * 8 + i;
+      const bool ok = kk < kend && col < ncol;
+      v[i] = sm[(size_t)(ok ? 2 * kk + gpar : 0) * ncol + (ok ? col : 0)];
+    }
+  };
+  auto put = [&](int j, const double (&v)[8]) {
+    unsigned char *b = lds + (j & 1) * kBuf;
+    const int col = (j0 + j) * 16 + c;
+    float f[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int kk = gs * 32 + k8 * 8 + i;
+      f[i] = kk < kend && col < ncol ? (float)v[i] : 0.0f;  // split as FP32: see split(float)
+    }
+    const int e = col_exponent<MODE>(f);
+    uint4 pc[3];
+    split8<MODE>(f, e, pc);
+    const int g = gpar * NSL + gs;
+    if constexpr (Form<MODE>::kScaled)
+      if (k8 == 0) reinterpret_cast<int *>(b + One::bt)[g * 16 + c] = e;
+    uint16_t *bt = reinterpret_cast<uint16_t *>(b);
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+      *reinterpret_cast<uint4 *>(bt + ((size_t)(g * NP + q) * 16 + c) * kRowPitch + k8 * 8) = pc[q];
+  };
+  const double ea0 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m]) : 1.0;
+  const double ea1 = Form<MODE>::kScaled ? ldexp(1.0, -lexp[2 * m + 1]) : 1.0;
+  float *ob = reinterpret_cast<float *>(lds + 2 * kBuf) + wave * (16 * kOutPitch);
+  // column tile j: its staged values in v, convert them, then (barrier) load
+  // tile j + PD into v and multiply tile j
+  auto step = [&](int j, double (&v)[8]) {
+    if (stg) put(j, v);
+    __syncthreads();
+    if (stg && j + PD < nj) load(j + PD, v);
+    if (mma) {
+      const unsigned char *b = lds + (j & 1) * kBuf;
+      inv_tile<MODE, NSL, NT, false>(reinterpret_cast<const uint16_t *>(b), reinterpret_cast<const int *>(b + One::bt),
+                                     ob, pa, ea0, ea1, nsl, four, geo, P2, ncol, m, wave, j0 + j, lane);
+    }
+  };
+  double v[PD][8];
+  if (stg) {
+#pragma unroll
+    for (int d = 0; d < PD; ++d)
+      if (d < nj) load(d, v[d]);
+  }
+  for (int j = 0; j < nj; j += PD) {
+#pragma unroll
+    for (int d = 0; d < PD; ++d)
+      if (j + d < nj) step(j + d, v[d]);
+  }
+}
+
 template <int MODE, int NSL>
 __global__ __launch_bounds__(256) void leg_fwd_mixed_wave_kernel(
     const float *__restrict__ four, double *__restrict__ spec, const uint16_t *__restrict__ L,
@@ -992,6 +1108,19 @@ int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, h
                   xcd_m_blocks(p, 0, (nct + 3) / 4), 256, 0, s, spec, four,
                   static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
                   p.xcd_mlen[0], p.zero, p.N, p.P2, p.P2p, ncol, nct, geo);
+    SHT_HIP(hipGetLastError());
+    return SHT_OK;
+  }
+  if (shape == SHT_SHAPE_STAGED) {
+    if (!mixed_stream_available(p)) return fail(SHT_ERR_UNSUPPORTED, "mixed inverse: streaming shape needs <= 128 latitude pairs");
+    // 8 column tiles per workgroup, loads 4 tiles ahead (T170, 776 fields,
+    // split-BF16 x6: 4 / 8 / 12 / 16 / 32 tiles 225 / 212 / 211 / 219 /
+    // 252 us; 2 / 3 / 4 tiles ahead 219 / 216 / 212 us)
+    constexpr int cs = 8;
+    launch_kernel(geo.stream ? leg_inv_mixed_stream_kernel<MODE, NSL, true, 4> : leg_inv_mixed_stream_kernel<MODE, NSL, false, 4>,
+                  xcd_m_blocks(p, 0, (nct + cs - 1) / cs), 512, 0, s, spec, four,
+                  static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],
+                  p.xcd_mlen[0], p.N, p.P2, p.P2p, ncol, nct, cs, geo);
     SHT_HIP(hipGetLastError());
     return SHT_OK;
   }

@@ -20,11 +20,13 @@ namespace {
 // FP64 plans: every shape but FUSED; mixed-precision plans: KSPLIT (shared
 // LDS data tile of one column tile), WAVE (data tile per wave in registers)
 // and TILE (shared LDS data tiles of two column tiles), and for the inverse
-// FUSED where the plan has it.
+// STAGED (table in registers, data streamed) and FUSED where the plan has
+// them.
 int nshapes(const Plan &p) { return p.prec == SHT_PREC_FP64 ? SHT_SHAPE_STAGED + 1 : 3; }
 
 bool shape_ok(const Plan &p, int s, int dir) {
   if (s == SHT_SHAPE_FUSED) return dir == 0 && fused_inv_available(p);
+  if (s == SHT_SHAPE_STAGED && p.prec != SHT_PREC_FP64) return dir == 0 && mixed_stream_available(p);
   return s == SHT_SHAPE_AUTO || (s >= 0 && s < nshapes(p));
 }
 
@@ -101,6 +103,7 @@ struct Candidate {
 std::vector<Candidate> candidates(const Plan &p, int dir, bool classes) {
   std::vector<Candidate> c;
   for (int s = 0; s < nshapes(p); ++s) c.push_back({s, KSplit{-1, -1}});
+  if (dir == 0 && p.prec != SHT_PREC_FP64 && mixed_stream_available(p)) c.push_back({SHT_SHAPE_STAGED, KSplit{-1, -1}});
   if (dir == 0 && fused_inv_available(p)) c.push_back({SHT_SHAPE_FUSED, KSplit{-1, -1}});
   if (!classes || p.prec != SHT_PREC_FP64 || p.N < 8) return c;
   const KSplit def = default_classes(p, dir);

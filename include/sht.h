@@ -345,8 +345,11 @@ enum {
   SHT_SHAPE_WAVE = 1,         /* wavefront per 16x16 tile, whole K, pipelined loads     */
   SHT_SHAPE_TILE = 2,         /* wavefront per 16 columns x all row tiles (registers)   */
   SHT_SHAPE_TILE_SPLIT = 3,   /* TILE with the K loop halved for the heaviest m         */
-  SHT_SHAPE_STAGED = 4,       /* workgroup per (m, 16 columns), data block in LDS, each
-                                 wave a (parity, row tile) over the whole K (FP64 only)  */
+  SHT_SHAPE_STAGED = 4,       /* FP64: workgroup per (m, 16 columns), data block in LDS,
+                                 each wave a (parity, row tile) over the whole K; mixed
+                                 inverse on grids of <= 128 latitude pairs: workgroup per
+                                 (m, 8 column tiles), the table held in registers while
+                                 the spectral tiles stream through LDS                   */
   SHT_SHAPE_FUSED = 5,        /* inverse of mixed plans with 64 <= im <= 512 and
                                  nmax < 192: KSPLIT's Legendre items and the ring FFTs
                                  in one persistent launch (grid values; gradients and
@@ -359,8 +362,8 @@ enum {
  * data tile shared in LDS), SHT_SHAPE_WAVE (wavefront per 16 columns, data
  * tile in registers) and SHT_SHAPE_TILE (workgroup per (m, 32 columns), each
  * wave's table fragments used for both column tiles), and for the inverse
- * SHT_SHAPE_FUSED where the plan has it (SHT_ERR_UNSUPPORTED otherwise);
- * their results are bitwise equal. */
+ * SHT_SHAPE_STAGED (jm <= 256) and SHT_SHAPE_FUSED where the plan has them
+ * (SHT_ERR_UNSUPPORTED otherwise); their results are bitwise equal. */
 int sht_plan_set_shapes(sht_plan *plan, int nf, int inv_shape, int fwd_shape);
 /* The shapes a call with nf fields would run (resolved, never SHT_SHAPE_AUTO). */
 int sht_plan_get_shapes(const sht_plan *plan, int nf, int *inv_shape, int *fwd_shape);

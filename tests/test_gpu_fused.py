@@ -155,7 +155,8 @@ def test_fused_in_autotuner_and_gradients():
     p = shtamd.Plan(N, im, jm, nf, precision=shtamd.PREC_SPLIT_BF16X6, flags=shtamd.FLAG_GRAD)
     (inv, fwd), (ti, tf) = p.autotune(nf, reps=5, shapes_only=True)
     say("autotune", inv, fwd, ti, tf)
-    assert inv in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE, shtamd.SHAPE_FUSED)
+    assert inv in (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE, shtamd.SHAPE_STAGED,
+                   shtamd.SHAPE_FUSED)
     p.set_shapes(0, shtamd.SHAPE_FUSED, shtamd.SHAPE_KSPLIT)
     spec = synth.spectra(N, nf, cfg=70)
     s = dev(shtamd.spec_to_real(spec))

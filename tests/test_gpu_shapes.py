@@ -206,13 +206,26 @@ def test_shape_argument_checks():
     m = shtamd.Plan(21, 64, 32, 8, precision=shtamd.PREC_SPLIT_BF16X6)
     with pytest.raises(shtamd.ShtError):
         m.set_shapes(0, shtamd.SHAPE_TILE_SPLIT, shtamd.SHAPE_TILE_SPLIT)
-    with pytest.raises(shtamd.ShtError):  # the staged shape is FP64 only
+    with pytest.raises(shtamd.ShtError):  # mixed plans: the staged shape is inverse only
         m.set_shapes(0, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_STAGED)
+    m.set_shapes(0, shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT)
+    assert m.get_shapes(5) == (shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT)
     m.set_shapes(0, shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     assert m.get_shapes(5) == (shtamd.SHAPE_WAVE, shtamd.SHAPE_KSPLIT)
     (inv, fwd), _ = m.autotune(8, reps=2)
-    assert inv in MIXED_SHAPES and fwd in MIXED_SHAPES
+    assert inv in MIXED_SHAPES + (shtamd.SHAPE_STAGED,) and fwd in MIXED_SHAPES
     m.close()
+    # the mixed staged (streaming) inverse holds the table of at most 128
+    # latitude pairs; the T170 grid's six-product inverse takes it from 300 fields
+    big = shtamd.Plan(106, 512, 288, 4, precision=shtamd.PREC_SPLIT_BF16X6)
+    with pytest.raises(shtamd.ShtError) as e:
+        big.set_shapes(0, shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT)
+    assert e.value.code == shtamd.SHT_ERR_UNSUPPORTED
+    big.close()
+    t170 = shtamd.Plan(170, 512, 256, 4, precision=shtamd.PREC_SPLIT_BF16X6)
+    assert t170.get_shapes(299)[0] == shtamd.SHAPE_TILE
+    assert t170.get_shapes(300)[0] == shtamd.SHAPE_STAGED
+    t170.close()
 
 
 MIXED_SHAPES = (shtamd.SHAPE_KSPLIT, shtamd.SHAPE_WAVE, shtamd.SHAPE_TILE)
@@ -220,21 +233,27 @@ MIXED_TOL = {shtamd.PREC_SPLIT_BF16: 5e-5, shtamd.PREC_SCALED_FP16: 1e-6, shtamd
 
 
 @pytest.mark.parametrize("prec", sorted(MIXED_TOL))
-@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (21, 64, 32, 9), (170, 512, 256, 7), (31, 64, 48, 17)])
+@pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (21, 64, 32, 9), (170, 512, 256, 7),
+                                        (170, 512, 256, 69), (31, 64, 48, 17)])
 def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
-    """The mixed-precision shapes (one or two column tiles per workgroup, or a
-    wave per tile) split the same operands into the same pieces and sum the
-    same products in the same order: bitwise equal results, within the form's
-    bound of the oracle.  The field counts give odd and even numbers of
-    16-column tiles, so the two-tile shape also runs a lone last tile."""
+    """The mixed-precision shapes (one or two column tiles per workgroup, a
+    wave per tile, or for the inverse the table held in registers while the
+    data stream through LDS) split the same operands into the same pieces and
+    sum the same products in the same order: bitwise equal results, within the
+    form's bound of the oracle.  The field counts give odd and even numbers of
+    16-column tiles, so the two-tile shape also runs a lone last tile, and the
+    streaming shape (8 column tiles per workgroup, loads 4 tiles ahead) runs a
+    single tile, a full chunk of 8 plus a lone tile (69 fields: 9 tiles) and
+    partial chunks with fewer tiles than its load distance."""
     say("mixed shapes", prec, N, nf)
     spec = synth.spectra(N, nf, cfg=14)
     ref = oracle.Plan(N, im, jm)
     g_ref = ref.spec_to_grid(spec)
     p = shtamd.Plan(N, im, jm, nf, precision=prec)
     out = []
-    for sh in MIXED_SHAPES:
-        p.set_shapes(0, sh, sh)
+    pairs = [(sh, sh) for sh in MIXED_SHAPES] + ([(shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT)] if jm <= 256 else [])
+    for inv, fwd in pairs:
+        p.set_shapes(0, inv, fwd)
         g = run_inverse(p, spec, nf)[0]
         s = run_forward(p, g_ref, nf)
         assert rel(g, g_ref) < MIXED_TOL[prec]
