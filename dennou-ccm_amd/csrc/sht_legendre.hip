@@ -1083,14 +1083,21 @@ int resolve_shape(const Plan &p, int nf, int dir) {
   if (d != SHT_SHAPE_AUTO) return d;
   // mixed forms: the shapes give bitwise equal results, so the choice is by
   // speed alone -- two column tiles per workgroup from 128 latitude pairs
-  // (T170: faster in both directions at 97 to 776 fields), one below; the
-  // six-product inverse streams from 300 fields on that grid (T170 inverse
-  // Legendre step, TILE against STAGED: 200 fields 49.0 / 52.8 us, 388
-  // fields 95.8 / 90.3, 776 fields 235 / 211, 3190 fields 872 / 792; the
-  // two-piece forms stay faster with TILE, 184 / 191 us at 776 fields)
+  // (T170: faster in both directions at 97 to 776 fields), one below.  On
+  // the 128-pair grid the inverse streams (STAGED): always in the
+  // six-product form, up to 8 fields in the two-piece forms, which run
+  // KSPLIT up to 24 fields.  T170 inverse Legendre step, KSPLIT / TILE /
+  // STAGED, us: x6 9 fields 8.5 / 10.7 / 8.1, 33 fields 15.9 / 14.2 /
+  // 13.6, 97 fields 30.9 / 29.7 / 25.9, 388 fields 107 / 98.8 / 82.7, 776
+  // fields 263 / 234 / 213; BF16 x3 4 fields 6.3 / 6.4 / 4.9, 17 fields
+  // 8.8 / 9.8 / 9.0, 97 fields 23.4 / 21.1 / 21.9, 776 fields 221 / 182 /
+  // 190.  Below 128 pairs the four idle table waves cost more than the
+  // stream saves (T85 x6, 97 fields: KSPLIT 8.3, STAGED 10.1 us).
   if (p.prec != SHT_PREC_FP64) {
-    if (dir == 0 && p.prec == SHT_PREC_SPLIT_BF16X6 && nf >= 300 && mixed_stream_available(p) && p.P2p >= 128)
-      return SHT_SHAPE_STAGED;
+    if (dir == 0 && p.P2p == 128 && mixed_stream_available(p)) {
+      if (p.prec == SHT_PREC_SPLIT_BF16X6 || nf <= 8) return SHT_SHAPE_STAGED;
+      if (nf <= 24) return SHT_SHAPE_KSPLIT;
+    }
     return p.P2p >= 128 ? SHT_SHAPE_TILE : SHT_SHAPE_KSPLIT;
   }
   // measured on MI355X from T21 to T170 and 16 to 388 fields: the staged

@@ -1099,6 +1099,19 @@ int build_mixed_mode(Plan &p) {
   return SHT_OK;
 }
 
+// Column tiles per workgroup of the streaming inverse: at most 8, and at
+// least 3 workgroups per m once there are more than 3 tiles, the chunks
+// balanced (one workgroup per CU runs, so a short last chunk or too few
+// workgroups leaves CUs idle).  Measured at T170 (split-BF16 x6): 776
+// fields 4 / 8 / 12 / 16 / 32 tiles 225 / 212 / 211 / 219 / 252 us; 200
+// fields (25 tiles) 8 tiles 52.1 us, 6 tiles 44.1 us; 33 fields (5 tiles)
+// 5 tiles 15.6 us, 2 tiles 13.6 us.
+int stream_chunk(int nct) {
+  if (nct <= 3) return nct;
+  const int nchunks = std::max(3, (nct + 7) / 8);
+  return (nct + nchunks - 1) / nchunks;
+}
+
 template <int MODE, int NSL>
 int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, hipStream_t s) {
   const int ncol = 2 * nf, nct = (ncol + 15) / 16;
@@ -1113,10 +1126,9 @@ int inv_nsl(const Plan &p, int shape, const double *spec, float *four, int nf, h
   }
   if (shape == SHT_SHAPE_STAGED) {
     if (!mixed_stream_available(p)) return fail(SHT_ERR_UNSUPPORTED, "mixed inverse: streaming shape needs <= 128 latitude pairs");
-    // 8 column tiles per workgroup, loads 4 tiles ahead (T170, 776 fields,
-    // split-BF16 x6: 4 / 8 / 12 / 16 / 32 tiles 225 / 212 / 211 / 219 /
-    // 252 us; 2 / 3 / 4 tiles ahead 219 / 216 / 212 us)
-    constexpr int cs = 8;
+    // loads 4 tiles ahead (T170, 776 fields, split-BF16 x6, 8 tiles per
+    // workgroup: 2 / 3 / 4 tiles ahead 219 / 216 / 212 us)
+    const int cs = stream_chunk(nct);
     launch_kernel(geo.stream ? leg_inv_mixed_stream_kernel<MODE, NSL, true, 4> : leg_inv_mixed_stream_kernel<MODE, NSL, false, 4>,
                   xcd_m_blocks(p, 0, (nct + cs - 1) / cs), 512, 0, s, spec, four,
                   static_cast<const uint16_t *>(p.Linv), p.loff_inv, p.lexp_inv, p.xcd_m[0],

@@ -216,15 +216,20 @@ def test_shape_argument_checks():
     assert inv in MIXED_SHAPES + (shtamd.SHAPE_STAGED,) and fwd in MIXED_SHAPES
     m.close()
     # the mixed staged (streaming) inverse holds the table of at most 128
-    # latitude pairs; the T170 grid's six-product inverse takes it from 300 fields
+    # latitude pairs; on the T170 grid it is the built-in inverse of the
+    # six-product form, and of the two-piece forms up to 8 fields
     big = shtamd.Plan(106, 512, 288, 4, precision=shtamd.PREC_SPLIT_BF16X6)
     with pytest.raises(shtamd.ShtError) as e:
         big.set_shapes(0, shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT)
     assert e.value.code == shtamd.SHT_ERR_UNSUPPORTED
+    assert big.get_shapes(97) == (shtamd.SHAPE_TILE, shtamd.SHAPE_TILE)
     big.close()
     t170 = shtamd.Plan(170, 512, 256, 4, precision=shtamd.PREC_SPLIT_BF16X6)
-    assert t170.get_shapes(299)[0] == shtamd.SHAPE_TILE
-    assert t170.get_shapes(300)[0] == shtamd.SHAPE_STAGED
+    assert [t170.get_shapes(nf)[0] for nf in (1, 97, 776)] == [shtamd.SHAPE_STAGED] * 3
+    t170.close()
+    t170 = shtamd.Plan(170, 512, 256, 4, precision=shtamd.PREC_SPLIT_BF16)
+    assert [t170.get_shapes(nf)[0] for nf in (8, 9, 24, 25)] == [
+        shtamd.SHAPE_STAGED, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_KSPLIT, shtamd.SHAPE_TILE]
     t170.close()
 
 
@@ -234,7 +239,7 @@ MIXED_TOL = {shtamd.PREC_SPLIT_BF16: 5e-5, shtamd.PREC_SCALED_FP16: 1e-6, shtamd
 
 @pytest.mark.parametrize("prec", sorted(MIXED_TOL))
 @pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (21, 64, 32, 9), (170, 512, 256, 7),
-                                        (170, 512, 256, 69), (31, 64, 48, 17)])
+                                        (170, 512, 256, 69), (170, 512, 256, 200), (31, 64, 48, 17)])
 def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
     """The mixed-precision shapes (one or two column tiles per workgroup, a
     wave per tile, or for the inverse the table held in registers while the
@@ -242,9 +247,10 @@ def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
     sum the same products in the same order: bitwise equal results, within the
     form's bound of the oracle.  The field counts give odd and even numbers of
     16-column tiles, so the two-tile shape also runs a lone last tile, and the
-    streaming shape (8 column tiles per workgroup, loads 4 tiles ahead) runs a
-    single tile, a full chunk of 8 plus a lone tile (69 fields: 9 tiles) and
-    partial chunks with fewer tiles than its load distance."""
+    streaming shape (up to 8 column tiles per workgroup, loads 4 tiles ahead)
+    runs single tiles (7 fields), chunks shorter than its load distance (69
+    fields: 3 x 3 tiles) and longer ones with a short last chunk (200 fields:
+    7 + 7 + 7 + 4 tiles)."""
     say("mixed shapes", prec, N, nf)
     spec = synth.spectra(N, nf, cfg=14)
     ref = oracle.Plan(N, im, jm)
