@@ -354,6 +354,77 @@ def run_small_workload(name: str, steps: int, cpu_seconds: float, seed_cfg: int)
     return out
 
 
+def run_t170_workload(nf: int, steps: int, seed_cfg: int = 5):
+    """The T170 round trip (512x256 Gaussian grid, the largest truncation of
+    the ensemble and serving mixes) at nf fields: FP64 and split-BF16 x6 with
+    the built-in shapes (eager, queue prefilled, events on the stream), the
+    per-kernel times from the stage timers, and for the split form the
+    inverse Legendre step in each of its inverse shapes.  Errors against the
+    FP64 oracle on the first 16 fields.  Returns a JSON-ready dict."""
+    import torch
+
+    import oracle
+    import shtamd
+    from shtamd import synth
+
+    n, im, jm = 170, 512, 256
+    spec_c = synth.spectra(n, nf, cfg=seed_cfg)
+    ref = oracle.Plan(n, im, jm)
+    k = min(nf, 16)
+    g_ref = ref.spec_to_grid(spec_c[:, :k])
+    out = {"metric": "T170 SHT fields/s (inverse + forward, 512x256 Gaussian grid)",
+           "workload": f"T170 round trip: {nf} fields, {im}x{jm} Gaussian grid", "fields": nf, "forms": {}}
+    s_in = torch.from_numpy(shtamd.spec_to_real(spec_c)).cuda()
+    grid = torch.empty((nf, jm, im), dtype=torch.float64, device="cuda")
+    s_out = torch.empty_like(s_in)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, reps):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(SPIN_CYCLES)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    for form in ("fp64", "bf16x6"):
+        plan = shtamd.Plan(n, im, jm, nf, precision=PRECISIONS[form])
+
+        def step():
+            plan.spec_to_grid(s_in, grid, nf, stream)
+            plan.grid_to_spec(grid, s_out, nf, stream)
+
+        ms = timed(step, steps)
+        plan.set_timing(True)
+        timed(step, max(1, steps // 4))
+        st = plan.stage_times()
+        plan.set_timing(False)
+        err = {"grid_rel_l2": rel(grid[:k].cpu().numpy(), g_ref),
+               "roundtrip_rel_l2": rel(shtamd.real_to_spec(s_out.cpu().numpy())[:, :k], spec_c[:, :k])}
+        f = {"fields_per_s": round(nf / (ms * 1e-3), 1), "ms_per_step": round(ms, 5),
+             "kernels_us": {key: round(v[0] / v[1] * 1e3, 2) for key, v in st.items() if v[1]},
+             "shapes": [SHAPE_NAMES[x] for x in plan.get_shapes(nf)], "err_vs_fp64_oracle": err}
+        if form != "fp64":
+            inv = {}
+            for name, sh in (("ksplit", shtamd.SHAPE_KSPLIT), ("tile", shtamd.SHAPE_TILE),
+                             ("staged", shtamd.SHAPE_STAGED)):
+                plan.set_shapes(nf, sh, shtamd.SHAPE_AUTO)
+                plan.set_timing(True)
+                timed(lambda: plan.spec_to_grid(s_in, grid, nf, stream), max(1, steps // 4))
+                v = plan.stage_times()["leg_inv"]
+                plan.set_timing(False)
+                inv[name] = round(v[0] / v[1] * 1e3, 2)
+            f["leg_inv_us_by_shape"] = inv
+        out["forms"][form] = f
+        plan.close()
+    return out
+
+
 def copy_calibration(nbytes: int = 64 << 20, reps: int = 3):
     """A plain device copy of known size: under rocprofv3 --pmc FETCH_SIZE and
     --pmc WRITE_SIZE, what the counters read per dispatch against the bytes
@@ -498,11 +569,15 @@ def main():
     ap.add_argument("--precision", default="auto", choices=sorted(PRECISIONS) + ["auto"],
                     help="auto: run every form; the headline is split-BF16 (six products, <=1e-6)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve", "copycal", "small"],
+    ap.add_argument("--workload", default="t85l32", choices=["t85l32", "ensemble", "serve", "copycal", "small",
+                                                              "t170"],
                     help="ensemble: the 64-member mixed-truncation batch; serve: request streams at "
                          "100-1000 req/s (separate JSON lines); copycal: a plain 64 MiB device copy, "
                          "three dispatches, the calibration run for rocprofv3's FETCH_SIZE/WRITE_SIZE; "
-                         "small: the T21L16 round trip and the T42L26 step with gradients, GPU and CPU oracle")
+                         "small: the T21L16 round trip and the T42L26 step with gradients, GPU and CPU oracle; "
+                         "t170: the T170 round trip at --fields fields (one JSON line each)")
+    ap.add_argument("--fields", default="145,776",
+                    help="field counts of the t170 workload (comma-separated)")
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests in the 500 req/s serving run of the headline line (0 = skip)")
     ap.add_argument("--members", type=int, default=64)
@@ -518,6 +593,10 @@ def main():
 
     if args.workload == "copycal":
         print(json.dumps(copy_calibration()))
+        return
+    if args.workload == "t170":
+        for nf in [int(x) for x in args.fields.split(",")]:
+            print(json.dumps(run_t170_workload(nf, max(1, args.steps // 10))), flush=True)
         return
     if args.workload == "small":
         for k, name in enumerate(SMALL_WORKLOADS):
