@@ -264,8 +264,8 @@ int sht_plan_create(const sht_desc *desc, sht_plan **out) {
   if (!rc) fft_fwd_capacity(p, p.fft_fwd_resident);
   if (!rc && p.prec != SHT_PREC_FP64) rc = build_mixed_tables(p);
   if (!rc) {
-    // [P2][nfg][N+1][2][2G]: field groups of G = 2048/im are allocated whole
-    const size_t four = (size_t)p.P2 * (p.N + 1) * 4 * (size_t)roundup(p.kmax, fft_group(p.im));
+    // [P2][nfg][N+1][2][2G]: field groups (G = max(2048/im, 8)) are allocated whole
+    const size_t four = (size_t)p.P2 * (p.N + 1) * 4 * (size_t)roundup(p.kmax, layout_group(fft_group(p.im)));
     rc = plan_alloc(p, reinterpret_cast<void **>(&p.four), sizeof(double) * four);
     if (!rc && (p.flags & SHT_FLAG_GRAD)) rc = plan_alloc(p, reinterpret_cast<void **>(&p.four2), sizeof(double) * four);
   }

@@ -39,7 +39,7 @@ struct FwdIO {
   int mode[2];  // 0 value, 1 divergence lon term (x i m / cos), 2 lat term (x 1 / cos)
 };
 
-// grid -> four (S, A halves).  Blocks: P2 * ceil(nf / G) x inputs.
+// grid -> four (S, A halves).  Blocks: the FFT tiles (four_tiles) x inputs.
 // DIRECT: the first FFT pass reads its butterflies' ring samples straight
 // from global memory (runs of L/R samples), which saves an LDS round trip and
 // wins while every block of the launch is resident at once; otherwise all
@@ -64,8 +64,8 @@ __global__ __launch_bounds__(kThreadsOf<C>) void fft_fwd_kernel(FwdIO<TF> io,
   __shared__ double2 buf[kPaddedOf<C>];
   __shared__ double2 twl[L];
   const int tile = spread ? (int)blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
-  const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
-  const int nfg = (nf + G - 1) / G;
+  const FourTile<G> ft(tile, P2);
+  const int p = ft.p, f0 = ft.fgr * G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const int tid = threadIdx.x;
   // the twiddles are loaded after the first ring loads are issued (one wait
@@ -117,9 +117,9 @@ __global__ __launch_bounds__(kThreadsOf<C>) void fft_fwd_kernel(FwdIO<TF> io,
       A = make_double2(-m * A.y, m * A.x);
     }
     if (m == 0) S.y = A.y = 0.0;
-    TF *o = four + ((((size_t)p * nfg + fgr) * M1 + m) * 2) * (2 * G) + 2 * g;
+    TF *o = four + ft.off(m, M1, nf) + 2 * g;
     store_pair(o, S);
-    store_pair(o + 2 * G, A);
+    store_pair(o + 2 * FourTile<G>::GL, A);
   }
 }
 
@@ -131,8 +131,8 @@ struct InvOutputs {
   int mode[3]; // 0 value, 1 GradLon, 2 GradLat
 };
 
-// four (even/odd Legendre sums) -> grid.  Blocks: P2 * ceil(nf / G) x noutputs,
-// one tile each (fft_inv_tile).
+// four (even/odd Legendre sums) -> grid.  Blocks: the FFT tiles (four_tiles)
+// x noutputs, one tile each (fft_inv_tile).
 template <int L, typename TF, int C>
 __global__ __launch_bounds__(kThreadsOf<C>) void fft_inv_kernel(InvOutputs<TF> io,
                                                                 const double2 *__restrict__ tw,
@@ -148,8 +148,7 @@ __global__ __launch_bounds__(kThreadsOf<C>) void fft_inv_kernel(InvOutputs<TF> i
 
 template <int L, int C, typename TF>
 int fwd_launch(const Plan &p, const FwdIO<TF> &io, int nin, int nf, hipStream_t s) {
-  constexpr int G = C / L;
-  const int nblk = p.P2 * ((nf + G - 1) / G);
+  const int nblk = four_tiles<C / L>(p.P2, nf);
   const bool resident = nblk * nin <= p.fft_fwd_resident[C == kFftSmall];
   auto kern = resident ? fft_fwd_kernel<L, TF, true, C> : fft_fwd_kernel<L, TF, false, C>;
   launch_kernel(kern, dim3(nblk, nin), kThreadsOf<C>, 0, s,
@@ -178,8 +177,7 @@ int fwd_resident(int device) {
 
 template <int L, int C, typename TF>
 int inv_launch(const Plan &p, const InvOutputs<TF> &io, int nout, int nf, hipStream_t s) {
-  constexpr int G = C / L;
-  const int nblk = p.P2 * ((nf + G - 1) / G);
+  const int nblk = four_tiles<C / L>(p.P2, nf);
   // same LDS and block size as the forward kernel, hence the same residency
   const bool resident = nblk * nout <= p.fft_fwd_resident[C == kFftSmall];
   launch_kernel(fft_inv_kernel<L, TF, C>, dim3(nblk, nout), kThreadsOf<C>, 0, s,

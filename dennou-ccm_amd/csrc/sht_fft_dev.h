@@ -99,6 +99,34 @@ __device__ __forceinline__ void dft_reg(double2 (&v)[R]) {
   for (int i = 0; i < R; ++i) v[i] = t[i];
 }
 
+// FFT tiles over the Fourier intermediate's field groups: a block of G fields
+// covers 1 / KS of a group of layout_group(G) fields (KS = 1 up to 256
+// longitudes).  Tile t is (pair p, FFT group fgr) with the KS blocks of one
+// group consecutive, so they run on one XCD at about the same time and the
+// parts of each 64-byte run meet in its L2.
+template <int G>
+struct FourTile {
+  static constexpr int GL = layout_group(G), KS = GL / G;
+  int p, fgr;
+  __device__ __forceinline__ FourTile(int tile, int P2) {
+    const int sub = tile % KS, rest = tile / KS;
+    p = rest % P2;
+    fgr = (rest / P2) * KS + sub;
+  }
+  // offset of (m, parity 0, the block's first field) for nf fields; parity
+  // 1 follows 2 GL values later
+  __device__ __forceinline__ size_t off(int m, int M1, int nf) const {
+    const int nfg = (nf + GL - 1) / GL;
+    return ((((size_t)p * nfg + fgr / KS) * M1 + m) * 2) * (2 * GL) + (fgr % KS) * (2 * G);
+  }
+};
+// FFT tiles of a call with nf fields
+template <int G>
+__host__ __device__ inline int four_tiles(int P2, int nf) {
+  constexpr int GL = layout_group(G);
+  return P2 * ((nf + GL - 1) / GL) * (GL / G);
+}
+
 // One Stockham pass of radix R over G transforms of length L, Ns the product
 // of the radices already applied.  Butterfly j of transform g reads
 // x[j + r L/R] (in(g, i) returns element i of transform g), applies
@@ -252,8 +280,8 @@ __device__ __forceinline__ void fft_inv_tile(double2 *buf, double2 *twl, const d
   constexpr int G = C / L;
   constexpr int kThreads = kThreadsOf<C>;
   constexpr int NM = (C / 2) / kThreads;  // max (m, field) pairs per thread
-  const int p = tile % P2, fgr = tile / P2, f0 = fgr * G;
-  const int nfg = (nf + G - 1) / G;
+  const FourTile<G> ft(tile, P2);
+  const int p = ft.p, f0 = ft.fgr * G;
   const int jn = P2 + p, js = P2 - 1 - p;
   const double rc = (mode == 0) ? 1.0 : 1.0 / coslat[jn];
   // every load issued unconditionally (pairs past M1 or nf read the tile's
@@ -267,9 +295,9 @@ __device__ __forceinline__ void fft_inv_tile(double2 *buf, double2 *twl, const d
     const int idx = tid + q * kThreads;
     const int m = idx / G, g = idx % G, f = f0 + g;
     const bool ok = m < M1 && f < nf;
-    const TF *e = src + ((((size_t)p * nfg + fgr) * M1 + (ok ? m : 0)) * 2) * (2 * G) + (ok ? 2 * g : 0);
+    const TF *e = src + ft.off(ok ? m : 0, M1, nf) + (ok ? 2 * g : 0);
     Er[q] = load_four_raw<COH>(e);
-    Or[q] = load_four_raw<COH>(e + 2 * G);
+    Or[q] = load_four_raw<COH>(e + 2 * FourTile<G>::GL);
   }
   if (tw)
     for (int q = tid; q < L; q += kThreads) twl[q] = tw[q];

@@ -44,9 +44,10 @@ int fail(int code, const std::string &msg);
 //
 //   four  [P2][nfg][N+1][par 2][2G]  the Fourier intermediate: per latitude pair
 //         and wavenumber the equatorially even/odd halves, complex with the
-//         fields of one FFT block (G = 2048/im fields, nfg = ceil(nf/G) groups)
-//         innermost, so each FFT block reads or writes one contiguous, aligned
-//         slice.  Forward: (F(mu)+F(-mu), F(mu)-F(-mu)); inverse: the even-n and
+//         fields of one group innermost (G = max(2048/im, 8) fields, nfg =
+//         ceil(nf/G) groups), so each FFT block reads or writes one contiguous,
+//         aligned slice (or, on grids above 256 longitudes, its part of each
+//         run).  Forward: (F(mu)+F(-mu), F(mu)-F(-mu)); inverse: the even-n and
 //         odd-n Legendre sums (north = even + odd, south = even - odd).
 //
 // Mixed-precision plans also hold the tables split into 16-bit pieces in the
@@ -172,15 +173,27 @@ __device__ __forceinline__ void put_spec(double *dst, double v) { __builtin_nont
 // still runs whole radix-16 butterflies per thread).
 constexpr int kFftBig = 2048, kFftSmall = 512;
 constexpr int kFftSmallMaxIm = 64, kFftSmallBelow = 256;
-inline int fft_group(int im) { return kFftBig / im; }  // the largest field group (allocation)
+inline int fft_group(int im) { return kFftBig / im; }  // the big blocks' field group
+// Field groups of the Fourier intermediate: the FFT block's G fields, but at
+// least kLayoutMinGroup, so that a (pair, m, parity) run holds at least 16
+// values (64 bytes in FP32) and the Legendre kernels, which write and read
+// across wavenumbers, move whole 64-byte pieces.  On grids of 512 to 2048
+// longitudes (G = 4, 2, 1) an FFT block then covers 1 / KS of a group.
+// Measured at T170 (512 longitudes), 776 fields, split-BF16 x6: the inverse
+// Legendre step 235 -> 213 us (TILE), 211 -> 191 us (STAGED) with groups of
+// 8 fields instead of 4.
+constexpr int kLayoutMinGroup = 8;
+__host__ __device__ constexpr int layout_group(int G) { return G >= kLayoutMinGroup ? G : kLayoutMinGroup; }
 inline int fft_block(const Plan &p, int nf) {
   const int g = kFftBig / p.im;
   return p.im <= kFftSmallMaxIm && (long)p.P2 * ((nf + g - 1) / g) < kFftSmallBelow ? kFftSmall : kFftBig;
 }
+// the layout for FFT blocks of G fields (groups of layout_group(G) fields)
 inline FourGeom four_geom_g(int N, int G, int nf) {
+  const int GL = layout_group(G);
   int lg = 0;
-  while ((1 << lg) < 2 * G) ++lg;
-  return FourGeom{N + 1, (nf + G - 1) / G, lg};
+  while ((1 << lg) < 2 * GL) ++lg;
+  return FourGeom{N + 1, (nf + GL - 1) / GL, lg};
 }
 // the layout a call with nf fields uses (its FFT block's field groups)
 inline FourGeom four_geom(const Plan &p, int nf) { return four_geom_g(p.N, fft_block(p, nf) / p.im, nf); }

@@ -537,7 +537,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void i
   int *const sy = a.sync;
   auto slot = [&](int k) { return sy + k * kSyncStride; };
   const int nL = a.mlen * a.nct;                // Legendre items per queue
-  const int nF = a.P2 * a.geo.nfg, nFx = (nF - x + 7) >> 3;
+  constexpr int GF = kLdsComplex / L;  // fields per FFT tile
+  const int nF = four_tiles<GF>(a.P2, a.nf), nFx = (nF - x + 7) >> 3;
   const int need = a.need;
   for (int q = tid; q < L; q += 256) twl[q] = a.tw[q];
   // Next item, chosen by thread 0: >= 0 Legendre item (queue << 24 | index),
@@ -585,9 +586,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void i
       if (i >= nFx) return kDone;
       pend = x + 8 * i;
     }
-    const int fgr = pend / a.P2;
-    const int c0 = (fgr << a.geo.lg) >> 4;
-    const int c1 = min((((fgr + 1) << a.geo.lg) - 1) >> 4, a.nct - 1);
+    const int f0 = FourTile<GF>(pend, a.P2).fgr * GF;  // the tile's columns 2 f0 .. 2 (f0 + GF) - 1
+    const int c0 = (2 * f0) >> 4;
+    const int c1 = min((2 * (f0 + GF) - 1) >> 4, a.nct - 1);
     const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
     for (int poll = 0;; ++poll) {
       // at most 4 column tiles per field group (2G <= 64 columns)
@@ -644,8 +645,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void i
       fft_inv_tile<L, float, true>(reinterpret_cast<double2 *>(lds), twl, nullptr, a.four, a.grid, 0,
                                    a.coslat, a.P2, a.N + 1, a.nf, a.jm, tile, ltid);
       if (bad) {
-        constexpr int G = kLdsComplex / L;
-        const int p = tile % a.P2, f0 = (tile / a.P2) * G;
+        constexpr int G = GF;
+        const FourTile<G> ft(tile, a.P2);
+        const int p = ft.p, f0 = ft.fgr * G;
         for (int q = tid; q < G * 2 * L; q += 256) {
           const int g = q / (2 * L), r = q % (2 * L), f = f0 + g;
           const int j = r < L ? a.P2 + p : a.P2 - 1 - p;
@@ -1330,7 +1332,8 @@ int launch_inv_fused(const Plan &p, const double *spec, float *four, double *gri
   a.need = p.N + 1 + (p.fused_inject ? 1 : 0);
   a.spin_ticks = p.fused_inject ? kFusedInjectedSpinTicks : kFusedSpinTicks;
   // no more workgroups than items, a multiple of 8 (one queue per XCD)
-  const long items = 8L * a.mlen * a.nct + (long)p.P2 * a.geo.nfg;
+  const int gf = fft_group(p.im);
+  const long items = 8L * a.mlen * a.nct + (long)p.P2 * a.geo.nfg * (layout_group(gf) / gf);
   const int nblk = (int)std::max(8L, std::min<long>(p.fused_blocks, items) / 8 * 8);
   return fused_dispatch(p, [&](auto l, auto mode, auto nsl) -> int {
     launch_kernel(inv_fused_kernel<decltype(l)::value, decltype(mode)::value, decltype(nsl)::value>, nblk, 256,
