@@ -355,7 +355,10 @@ __device__ __forceinline__ void inv_tile(const uint16_t *bt, const int *ebits, f
       const f4 v = *reinterpret_cast<const f4 *>(ob + pl * kOutPitch + par * 16 + c4);
       if (p < P2 && cc < cend) put_four4<NT, COH>(four + geo.at(p, m, par, cc), v);
     }
-  } else if (col < ncol) {  // field groups of fewer than 4 columns (im = 2048)
+  } else if (col < ncol) {
+    // field groups of fewer than 4 columns: none since groups hold at least
+    // 8 fields (layout_group), but without this path the compiler spills 94
+    // registers of the T85 six-product inverse (2 with it)
 #pragma unroll
     for (int par = 0; par < 2; ++par)
 #pragma unroll

@@ -266,9 +266,10 @@ def test_error_codes():
 @pytest.mark.parametrize("prec,tol", [(shtamd.PREC_FP64, TOL), (shtamd.PREC_SPLIT_BF16X6, 1e-6)])
 def test_long_rings(im, prec, tol):
     """Ring lengths 1024 and 2048 (three FFT passes, G = 2 and 1 fields per
-    FFT block); at 2048 a field group is narrower than the 4 columns the mixed
-    inverse stores at once, so it takes the column-by-column store path.
-    Gradients too (the derivative table and the GradLon/GradLat FFT modes)."""
+    FFT block, a quarter and an eighth of the Fourier intermediate's 8-field
+    groups; with 7 fields the one group is partly filled and the last
+    blocks hold no field).  Gradients too (the derivative table and the
+    GradLon/GradLat FFT modes)."""
     N, jm, nf = 42, 64, 7
     say("long rings", im, prec)
     spec = synth.spectra(N, nf, cfg=16)
