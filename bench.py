@@ -587,7 +587,7 @@ def main():
                          "fused: the built-in choice with the fused inverse for the mixed forms")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported "
+        print(json.dumps({"error": "the transform runs on one GPU; --gpus above 1 is not supported "
                                    "(the transform would shard by field batch, one slab of levels per GPU)"}))
         sys.exit(2)
 
