@@ -112,7 +112,7 @@ int sht_spec_to_grid_dev(sht_plan *plan, const double *spec, double *grid,
  * The Fourier intermediate goes to a static 32 MiB device array per GPU
  * (looked up at the first sht_plan_create on that device) instead of the
  * plan's scratch; calls whose intermediate (jm/2 * ceil(nfield/G) * (nmax+1)
- * * 4G values, G = 2048/im, 8 bytes each in FP64 plans, 4 in the mixed
+ * * 4G values, G = max(2048/im, 8), 8 bytes each in FP64 plans, 4 in the mixed
  * forms) exceeds it fail with SHT_ERR_INVALID.  All calls on one GPU share the
  * array: calls that overlap on one GPU race.  Results equal
  * sht_spec_to_grid_dev's bitwise. */
