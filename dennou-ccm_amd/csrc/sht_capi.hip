@@ -69,7 +69,10 @@ static int ensure_host_staging(Plan &p) {
 // (at most 8) so that the PCIe copy of one chunk overlaps the host copy of
 // another: in, each chunk's host -> pinned copy is followed by its async
 // upload; out, the downloads are queued behind the transform and each
-// chunk's pinned -> host copy starts when its download has landed.
+// chunk's pinned -> host copy starts when its download has landed, split
+// over worker threads (host::parallel_copy: one thread could not keep up
+// with the next chunk's transfer; the inbound copies gained nothing from
+// more threads).
 constexpr size_t kHostChunkBytes = (size_t)2 << 20;
 
 static int host_chunks(size_t bytes) {
@@ -97,7 +100,7 @@ static int download_chunked(Plan &p, double *out, size_t n, hipStream_t s) {
   for (int c = 0; c < nc; ++c) {
     const size_t a = n * c / nc, b = n * (c + 1) / nc;
     SHT_HIP(hipEventSynchronize(p.host_events[c]));
-    std::memcpy(out + a, p.pinned + a, sizeof(double) * (b - a));
+    host::parallel_copy(out + a, p.pinned + a, sizeof(double) * (b - a));
   }
   return SHT_OK;
 }

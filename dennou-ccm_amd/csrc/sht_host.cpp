@@ -5,8 +5,12 @@
 #include <cerrno>
 #include <climits>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <thread>
 
 namespace sht {
 namespace host {
@@ -238,6 +242,76 @@ int injected_count(const char *name) {
   const long x = std::strtol(v, &end, 10);
   if (*end != '\0' || x < 1 || x > INT_MAX) return 0;
   return (int)x;
+}
+
+// Workers of parallel_copy.  Job g: worker i copies part i + 1 of `parts`
+// (the caller part 0) and counts itself done; the next job starts only when
+// every worker has finished this one, so each worker sees every generation.
+namespace {
+struct CopyPool {
+  std::mutex m;
+  std::condition_variable start, done;
+  std::mutex busy;  // held by the caller of the running parallel copy
+  int nworkers = 0;
+  unsigned gen = 0;
+  char *dst = nullptr;
+  const char *src = nullptr;
+  size_t bytes = 0;
+  int parts = 1, pending = 0;
+
+  void worker(int id) {
+    unsigned seen = 0;
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+      start.wait(lk, [&] { return gen != seen; });
+      seen = gen;
+      char *d = dst;
+      const char *s = src;
+      const size_t a = bytes * (id + 1) / parts, b = bytes * (id + 2) / parts;
+      lk.unlock();
+      std::memcpy(d + a, s + a, b - a);
+      lk.lock();
+      if (--pending == 0) done.notify_one();
+    }
+  }
+};
+
+CopyPool *copy_pool() {
+  // never freed: the detached workers wait on it until the process ends
+  static CopyPool *pool = [] {
+    auto *q = new CopyPool;
+    const unsigned hw = std::thread::hardware_concurrency();
+    q->nworkers = hw > 1 ? (int)std::min<unsigned>(kCopyWorkers, hw - 1) : 0;
+    for (int i = 0; i < q->nworkers; ++i) std::thread(&CopyPool::worker, q, i).detach();
+    return q;
+  }();
+  return pool;
+}
+}  // namespace
+
+void parallel_copy(void *dst, const void *src, size_t bytes) {
+  CopyPool *p = bytes >= kParallelCopyMin ? copy_pool() : nullptr;
+  if (!p || p->nworkers == 0 || !p->busy.try_lock()) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const int parts = p->nworkers + 1;
+  {
+    std::lock_guard<std::mutex> lk(p->m);
+    p->dst = static_cast<char *>(dst);
+    p->src = static_cast<const char *>(src);
+    p->bytes = bytes;
+    p->parts = parts;
+    p->pending = p->nworkers;
+    ++p->gen;
+  }
+  p->start.notify_all();
+  std::memcpy(dst, src, bytes / parts);
+  {
+    std::unique_lock<std::mutex> lk(p->m);
+    p->done.wait(lk, [&] { return p->pending == 0; });
+  }
+  p->busy.unlock();
 }
 
 }  // namespace host

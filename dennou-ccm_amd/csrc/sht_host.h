@@ -103,6 +103,18 @@ int restart_write(const char *path, const RestartDims &d, double time, const dou
                   const double *sig, const double *const *fields, std::string &msg);
 int restart_read(const char *path, const RestartDims &d, double *time, double *const *fields, std::string &msg);
 
+// ---- copies between a caller's host arrays and pinned staging memory (the
+// host drop-in entry points): the calling thread and up to kCopyWorkers
+// persistent worker threads each copy a contiguous part.  One such copy runs
+// in parallel at a time; a caller that finds the workers busy, or a copy
+// below kParallelCopyMin bytes, copies alone.  The workers live until the
+// process ends.  Measured on the MI355X host (16 cores, 25.4 MB grid in 8
+// chunks, each copied out of pinned memory while the next chunk's PCIe
+// transfer runs): 0.93 ms with one thread, 0.62 ms with four.
+constexpr int kCopyWorkers = 3;
+constexpr size_t kParallelCopyMin = (size_t)1 << 20;
+void parallel_copy(void *dst, const void *src, size_t bytes);
+
 // ---- test hook: the value of environment variable `name` as a count
 // (0 when unset or not a positive integer).  SHT_INJECT_PLAN_FAILURE=k makes
 // the k-th device allocation of each sht_plan_create fail (SHT_ERR_NOMEM);

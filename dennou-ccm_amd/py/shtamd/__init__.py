@@ -331,13 +331,19 @@ class Plan:
         check(lib().sht_plan_autotune_flags(self._h, nfield, reps, path, TUNE_SHAPES_ONLY if shapes_only else 0, us))
         return self.get_shapes(nfield), (us[0], us[1])
 
-    # ---- host drop-in (SPML packing), numpy arrays
-    def _host_call(self, fn, inp, in_tail, out_tail):
+    # ---- host drop-in (SPML packing), numpy arrays; out= takes a preallocated
+    # result array (as a model passes its own arrays), else one is allocated
+    def _host_call(self, fn, inp, in_tail, out_tail, out=None):
         inp = np.ascontiguousarray(inp, dtype=np.float64)
         if inp.ndim != 1 + len(in_tail) or inp.shape[1:] != tuple(in_tail):
             raise ValueError(f"expected an array of shape (nlev, {', '.join(map(str, in_tail))}), "
                              f"got {inp.shape}")
-        out = np.empty((inp.shape[0], *out_tail))
+        shape = (inp.shape[0], *out_tail)
+        if out is None:
+            out = np.empty(shape)
+        elif (not isinstance(out, np.ndarray) or out.dtype != np.float64 or out.shape != shape
+              or not out.flags.c_contiguous or not out.flags.writeable):
+            raise ValueError(f"out must be a writeable C-contiguous float64 array of shape {shape}")
         check(fn(self._h, _host(inp), _host(out), inp.shape[0]))
         return out
 
@@ -347,29 +353,29 @@ class Plan:
     def _xy(self):
         return (self.jm, self.im)
 
-    def xya_wa(self, wa):
-        return self._host_call(lib().sht_xya_wa, wa, self._wa(), self._xy())
+    def xya_wa(self, wa, out=None):
+        return self._host_call(lib().sht_xya_wa, wa, self._wa(), self._xy(), out)
 
-    def wa_xya(self, xya):
-        return self._host_call(lib().sht_wa_xya, xya, self._xy(), self._wa())
+    def wa_xya(self, xya, out=None):
+        return self._host_call(lib().sht_wa_xya, xya, self._xy(), self._wa(), out)
 
-    def xya_gradlon_wa(self, wa):
-        return self._host_call(lib().sht_xya_gradlon_wa, wa, self._wa(), self._xy())
+    def xya_gradlon_wa(self, wa, out=None):
+        return self._host_call(lib().sht_xya_gradlon_wa, wa, self._wa(), self._xy(), out)
 
-    def xya_gradlat_wa(self, wa):
-        return self._host_call(lib().sht_xya_gradlat_wa, wa, self._wa(), self._xy())
+    def xya_gradlat_wa(self, wa, out=None):
+        return self._host_call(lib().sht_xya_gradlat_wa, wa, self._wa(), self._xy(), out)
 
-    def wa_divlon_xya(self, xya):
-        return self._host_call(lib().sht_wa_divlon_xya, xya, self._xy(), self._wa())
+    def wa_divlon_xya(self, xya, out=None):
+        return self._host_call(lib().sht_wa_divlon_xya, xya, self._xy(), self._wa(), out)
 
-    def wa_divlat_xya(self, xya):
-        return self._host_call(lib().sht_wa_divlat_xya, xya, self._xy(), self._wa())
+    def wa_divlat_xya(self, xya, out=None):
+        return self._host_call(lib().sht_wa_divlat_xya, xya, self._xy(), self._wa(), out)
 
-    def wa_lapla_wa(self, wa):
-        return self._host_call(lib().sht_wa_lapla_wa, wa, self._wa(), self._wa())
+    def wa_lapla_wa(self, wa, out=None):
+        return self._host_call(lib().sht_wa_lapla_wa, wa, self._wa(), self._wa(), out)
 
-    def wa_laplainv_wa(self, wa):
-        return self._host_call(lib().sht_wa_laplainv_wa, wa, self._wa(), self._wa())
+    def wa_laplainv_wa(self, wa, out=None):
+        return self._host_call(lib().sht_wa_laplainv_wa, wa, self._wa(), self._wa(), out)
 
 
 class Zonal:

@@ -284,7 +284,39 @@ static void restart() {
   CHECK(restart_write("/nonexistent-dir/x.nc", d, 0.0, lon, lat, sig, in.data(), msg) == SHT_ERR_INVALID);
 }
 
+// parallel_copy: every size (below and above the parallel threshold, sizes
+// that do not divide into the parts), bytes outside the range untouched, and
+// four threads copying at once (one runs in parallel, the others alone)
+static void copies() {
+  std::mt19937_64 rng(11);
+  const size_t sizes[] = {0, 1, 7, kParallelCopyMin - 1, kParallelCopyMin, kParallelCopyMin + 3,
+                          (size_t)7 * kParallelCopyMin + 5};
+  for (size_t n : sizes) {
+    std::vector<unsigned char> src(n + 16), dst(n + 16, 0xAB);
+    for (auto &b : src) b = (unsigned char)rng();
+    parallel_copy(dst.data() + 8, src.data() + 8, n);
+    bool ok = true;
+    for (size_t i = 0; i < n + 16; ++i) ok &= dst[i] == ((i >= 8 && i < n + 8) ? src[i] : 0xAB);
+    CHECK(ok);
+  }
+  std::atomic<int> bad{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < 4; ++t)
+    th.emplace_back([&, t] {
+      const size_t n = (size_t)3 * kParallelCopyMin + 17 * t;
+      std::vector<unsigned char> a(n), b(n);
+      for (int rep = 0; rep < 20; ++rep) {
+        for (size_t i = 0; i < n; ++i) a[i] = (unsigned char)(i * 7 + rep + t);
+        parallel_copy(b.data(), a.data(), n);
+        if (a != b) ++bad;
+      }
+    });
+  for (auto &x : th) x.join();
+  CHECK(bad == 0);
+}
+
 int main() {
+  copies();
   descriptors();
   indices();
   tune_cache();

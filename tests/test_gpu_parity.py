@@ -187,6 +187,10 @@ def test_host_dropin_spml_packing():
     assert rel(p.xya_gradlon_wa(wa), ref.spec_to_grid(spec, kind=1)) < TOL
     assert rel(p.xya_gradlat_wa(wa), ref.spec_to_grid(spec, kind=2)) < TOL
     assert rel(p.wa_xya(xya), wa) < TOL
+    # into the caller's own arrays (out=), bitwise the same results
+    xya2, wa2 = np.full_like(xya, 7.0), np.full_like(wa, 7.0)
+    assert p.xya_wa(wa, out=xya2) is xya2 and np.array_equal(xya2, xya)
+    assert p.wa_xya(xya, out=wa2) is wa2 and np.array_equal(wa2, p.wa_xya(xya))
     # device layout conversions
     wa_d = dev(wa)
     sc = torch.empty((p.K, nlev, 2), dtype=torch.float64, device="cuda")
@@ -259,6 +263,11 @@ def test_error_codes():
         p.wa_lapla_wa(np.zeros(22 * 22))  # 1-D: no level axis
     with pytest.raises(ValueError):
         p.xya_wa(np.zeros((2, 22 * 22 + 1)))
+    wa = np.zeros((2, 22 * 22))
+    for bad in (np.zeros((2, 32, 63)), np.zeros((2, 32, 64), dtype=np.float32),
+                np.zeros((2, 64, 32)).transpose(0, 2, 1), np.zeros((3, 32, 64))):
+        with pytest.raises(ValueError):
+            p.xya_wa(wa, out=bad)  # wrong shape, dtype, layout or level count
     p.close()
 
 
