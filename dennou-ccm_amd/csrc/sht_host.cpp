@@ -12,6 +12,8 @@
 #include <mutex>
 #include <thread>
 
+#include <unistd.h>
+
 namespace sht {
 namespace host {
 
@@ -249,6 +251,7 @@ int injected_count(const char *name) {
 // every worker has finished this one, so each worker sees every generation.
 namespace {
 struct CopyPool {
+  pid_t pid = 0;  // the process that started the workers (a forked child has none)
   std::mutex m;
   std::condition_variable start, done;
   std::mutex busy;  // held by the caller of the running parallel copy
@@ -280,6 +283,7 @@ CopyPool *copy_pool() {
   // never freed: the detached workers wait on it until the process ends
   static CopyPool *pool = [] {
     auto *q = new CopyPool;
+    q->pid = getpid();
     const unsigned hw = std::thread::hardware_concurrency();
     q->nworkers = hw > 1 ? (int)std::min<unsigned>(kCopyWorkers, hw - 1) : 0;
     for (int i = 0; i < q->nworkers; ++i) std::thread(&CopyPool::worker, q, i).detach();
@@ -291,7 +295,7 @@ CopyPool *copy_pool() {
 
 void parallel_copy(void *dst, const void *src, size_t bytes) {
   CopyPool *p = bytes >= kParallelCopyMin ? copy_pool() : nullptr;
-  if (!p || p->nworkers == 0 || !p->busy.try_lock()) {
+  if (!p || p->nworkers == 0 || p->pid != getpid() || !p->busy.try_lock()) {
     std::memcpy(dst, src, bytes);
     return;
   }

@@ -107,7 +107,8 @@ int restart_read(const char *path, const RestartDims &d, double *time, double *c
 // host drop-in entry points): the calling thread and up to kCopyWorkers
 // persistent worker threads each copy a contiguous part.  One such copy runs
 // in parallel at a time; a caller that finds the workers busy, or a copy
-// below kParallelCopyMin bytes, copies alone.  The workers live until the
+// below kParallelCopyMin bytes, copies alone, and so does a forked child
+// process (the workers are not forked with it).  The workers live until the
 // process ends.  Measured on the MI355X host (16 cores, 25.4 MB grid in 8
 // chunks, each copied out of pinned memory while the next chunk's PCIe
 // transfer runs): 0.93 ms with one thread, 0.62 ms with four.

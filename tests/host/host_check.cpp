@@ -16,6 +16,7 @@
 #include <thread>
 #include <vector>
 
+#include <sys/wait.h>
 #include <unistd.h>
 
 #include "sht_host.h"
@@ -313,6 +314,17 @@ static void copies() {
     });
   for (auto &x : th) x.join();
   CHECK(bad == 0);
+  // a forked child (no workers of its own) still copies, alone
+  std::fflush(stdout);
+  const pid_t pid = fork();
+  if (pid == 0) {
+    const size_t n = (size_t)2 * kParallelCopyMin + 1;
+    std::vector<unsigned char> a(n, 3), b(n, 0);
+    parallel_copy(b.data(), a.data(), n);
+    _exit(a == b ? 0 : 1);
+  }
+  int status = -1;
+  CHECK(pid > 0 && waitpid(pid, &status, 0) == pid && WIFEXITED(status) && WEXITSTATUS(status) == 0);
 }
 
 int main() {
