@@ -851,9 +851,12 @@ __global__ __launch_bounds__(256) void leg_inv_mixed_wave_kernel(
 // table fragments of row tile w for the whole chunk, so each m's table is
 // read once per chunk instead of once per one or two column tiles.  The
 // spectral tiles pass through two LDS buffers: wave w stages group w of each
-// column tile (2 nsl <= 8 groups: parity, K slice) from registers that are
-// loaded PD tiles ahead, so while the waves multiply column tile j the loads
-// of tiles j + 1 .. j + PD are in flight.  Per output tile the products,
+// column tile (2 nsl <= 8 groups: parity, K slice) from registers loaded PD
+// tiles ahead.  (The compiler's wait counting has each tile's conversion wait
+// for every earlier load, so fewer tiles stay in flight than issued; forcing
+// exact counts -- the table awaited first, no branch around the loads, the
+// steps unrolled -- measured slower below 776 fields and equal at 776.)
+// Per output tile the products,
 // their order and the stores are those of leg_inv_mixed_item, so the results
 // equal the other shapes' bitwise.  One workgroup per CU (the table takes
 // 72 of the six-product form's ~210 registers at 128 pairs).
