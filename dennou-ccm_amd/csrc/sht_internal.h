@@ -179,9 +179,10 @@ inline int fft_group(int im) { return kFftBig / im; }  // the big blocks' field 
 // values (64 bytes in FP32) and the Legendre kernels, which write and read
 // across wavenumbers, move whole 64-byte pieces.  On grids of 512 to 2048
 // longitudes (G = 4, 2, 1) an FFT block then covers 1 / KS of a group.
-// Measured at T170 (512 longitudes), 776 fields, split-BF16 x6: the inverse
-// Legendre step 235 -> 213 us (TILE), 211 -> 191 us (STAGED) with groups of
-// 8 fields instead of 4.
+// Measured at T170 (512 longitudes), 776 fields, split-BF16 x6, groups of 8
+// fields instead of 4: inverse Legendre 214 -> 185-189 us, forward Legendre
+// 187 -> 173-175 us, forward FFT 231 -> 238-241 us (its blocks store half
+// runs), round trip 896 -> 863-866 us.
 constexpr int kLayoutMinGroup = 8;
 __host__ __device__ constexpr int layout_group(int G) { return G >= kLayoutMinGroup ? G : kLayoutMinGroup; }
 inline int fft_block(const Plan &p, int nf) {
