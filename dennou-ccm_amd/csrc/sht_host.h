@@ -111,7 +111,9 @@ int restart_read(const char *path, const RestartDims &d, double *time, double *c
 // process (the workers are not forked with it).  The workers live until the
 // process ends.  Measured on the MI355X host (16 cores, 25.4 MB grid in 8
 // chunks, each copied out of pinned memory while the next chunk's PCIe
-// transfer runs): 0.93 ms with one thread, 0.62 ms with four.
+// transfer runs): 0.93 ms with one thread, 0.62 ms with four; into pinned
+// memory the gain depends on the machine (none on one, 1.19 -> 0.80 ms for
+// the whole call on another).
 constexpr int kCopyWorkers = 3;
 constexpr size_t kParallelCopyMin = (size_t)1 << 20;
 void parallel_copy(void *dst, const void *src, size_t bytes);
