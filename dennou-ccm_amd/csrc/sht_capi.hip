@@ -66,7 +66,7 @@ static int ensure_host_staging(Plan &p) {
 }
 
 // The host copies of a drop-in call, in chunks of at least kHostChunkBytes
-// (at most 8) so that the PCIe copy of one chunk overlaps the host copy of
+// (at most kHostMaxChunks) so that the PCIe copy of one chunk overlaps the host copy of
 // another: in, each chunk's host -> pinned copy is followed by its async
 // upload; out, the downloads are queued behind the transform and each
 // chunk's pinned -> host copy starts when its download has landed.  Both
@@ -75,10 +75,14 @@ static int ensure_host_staging(Plan &p) {
 // (T85L32, 97 levels: xya_wa 1.22 -> 0.81 ms, wa_xya 0.92-1.19 -> 0.79-0.81
 // ms).
 constexpr size_t kHostChunkBytes = (size_t)2 << 20;
+// with the copies split over threads, fewer and larger chunks pay less
+// per-transfer overhead (97 levels, 25 MB grid: at most 8 / 6 / 4 / 3
+// chunks, wa_xya 0.79-0.81 / 0.75 / 0.74-0.75 / 0.76-0.77 ms)
+constexpr size_t kHostMaxChunks = 4;
 
 static int host_chunks(size_t bytes) {
   const size_t n = bytes / kHostChunkBytes;
-  return n < 1 ? 1 : n > 8 ? 8 : (int)n;
+  return n < 1 ? 1 : n > kHostMaxChunks ? kHostMaxChunks : (int)n;
 }
 
 static int upload_chunked(Plan &p, const double *in, size_t n, hipStream_t s) {
