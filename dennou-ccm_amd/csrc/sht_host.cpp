@@ -285,8 +285,17 @@ CopyPool *copy_pool() {
     auto *q = new CopyPool;
     q->pid = getpid();
     const unsigned hw = std::thread::hardware_concurrency();
-    q->nworkers = hw > 1 ? (int)std::min<unsigned>(kCopyWorkers, hw - 1) : 0;
-    for (int i = 0; i < q->nworkers; ++i) std::thread(&CopyPool::worker, q, i).detach();
+    const int want = hw > 1 ? (int)std::min<unsigned>(kCopyWorkers, hw - 1) : 0;
+    // a worker that cannot be started (thread limits) leaves fewer workers;
+    // none at all leaves plain copies
+    for (int i = 0; i < want; ++i) {
+      try {
+        std::thread(&CopyPool::worker, q, i).detach();
+      } catch (...) {
+        break;
+      }
+      q->nworkers = i + 1;
+    }
     return q;
   }();
   return pool;
