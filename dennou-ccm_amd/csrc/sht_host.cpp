@@ -2,6 +2,7 @@
 #include "sht_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <climits>
 #include <cmath>
@@ -246,9 +247,13 @@ int injected_count(const char *name) {
   return (int)x;
 }
 
-// Workers of parallel_copy.  Job g: worker i copies part i + 1 of `parts`
-// (the caller part 0) and counts itself done; the next job starts only when
-// every worker has finished this one, so each worker sees every generation.
+// Workers of parallel_copy.  A copy is cut into parts that the caller and
+// every worker that joins take from a shared counter, so a worker that is
+// slow to be scheduled (the host's cores busy with the caller's own threads)
+// only leaves more parts to the others: the caller never waits for a worker
+// that has not started, only for parts already taken.  A worker joins a job
+// while it is open; the caller closes it once no part is left and waits for
+// the workers inside it to leave.
 namespace {
 struct CopyPool {
   pid_t pid = 0;  // the process that started the workers (a forked child has none)
@@ -257,24 +262,34 @@ struct CopyPool {
   std::mutex busy;  // held by the caller of the running parallel copy
   int nworkers = 0;
   unsigned gen = 0;
+  bool open = false;
+  int active = 0;  // workers inside the open job
   char *dst = nullptr;
   const char *src = nullptr;
   size_t bytes = 0;
-  int parts = 1, pending = 0;
+  int nparts = 1;
+  std::atomic<int> next{0};
 
-  void worker(int id) {
+  void run_parts() {
+    for (;;) {
+      const int i = next.fetch_add(1, std::memory_order_relaxed);
+      if (i >= nparts) return;
+      const size_t a = bytes * i / nparts, b = bytes * (i + 1) / nparts;
+      std::memcpy(dst + a, src + a, b - a);
+    }
+  }
+  void worker() {
     unsigned seen = 0;
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
       start.wait(lk, [&] { return gen != seen; });
       seen = gen;
-      char *d = dst;
-      const char *s = src;
-      const size_t a = bytes * (id + 1) / parts, b = bytes * (id + 2) / parts;
+      if (!open) continue;
+      ++active;
       lk.unlock();
-      std::memcpy(d + a, s + a, b - a);
+      run_parts();
       lk.lock();
-      if (--pending == 0) done.notify_one();
+      if (--active == 0) done.notify_all();
     }
   }
 };
@@ -290,7 +305,7 @@ CopyPool *copy_pool() {
     // none at all leaves plain copies
     for (int i = 0; i < want; ++i) {
       try {
-        std::thread(&CopyPool::worker, q, i).detach();
+        std::thread(&CopyPool::worker, q).detach();
       } catch (...) {
         break;
       }
@@ -308,21 +323,22 @@ void parallel_copy(void *dst, const void *src, size_t bytes) {
     std::memcpy(dst, src, bytes);
     return;
   }
-  const int parts = p->nworkers + 1;
   {
     std::lock_guard<std::mutex> lk(p->m);
     p->dst = static_cast<char *>(dst);
     p->src = static_cast<const char *>(src);
     p->bytes = bytes;
-    p->parts = parts;
-    p->pending = p->nworkers;
+    p->nparts = 4 * (p->nworkers + 1);
+    p->next.store(0, std::memory_order_relaxed);
+    p->open = true;
     ++p->gen;
   }
   p->start.notify_all();
-  std::memcpy(dst, src, bytes / parts);
+  p->run_parts();
   {
     std::unique_lock<std::mutex> lk(p->m);
-    p->done.wait(lk, [&] { return p->pending == 0; });
+    p->open = false;
+    p->done.wait(lk, [&] { return p->active == 0; });
   }
   p->busy.unlock();
 }

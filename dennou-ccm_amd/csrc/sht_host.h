@@ -105,8 +105,10 @@ int restart_read(const char *path, const RestartDims &d, double *time, double *c
 
 // ---- copies between a caller's host arrays and pinned staging memory (the
 // host drop-in entry points): the calling thread and up to kCopyWorkers
-// persistent worker threads each copy a contiguous part.  One such copy runs
-// in parallel at a time; a caller that finds the workers busy, or a copy
+// persistent worker threads take parts of the range from a shared counter
+// (a worker the scheduler has not run yet leaves its parts to the others;
+// the caller never waits for it).  One such copy runs in parallel at a
+// time; a caller that finds the workers busy, or a copy
 // below kParallelCopyMin bytes, copies alone, and so does a forked child
 // process (the workers are not forked with it).  The workers live until the
 // process ends.  Measured on the MI355X host (16 cores, 25.4 MB grid in 8
