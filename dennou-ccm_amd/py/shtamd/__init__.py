@@ -145,12 +145,28 @@ SIGNATURES = {
 STAGES = ("leg_inv", "fft_inv", "fft_fwd", "leg_fwd")
 
 
+def _init_torch_device():
+    """PyTorch ships its own HIP runtime beside the system one this library
+    links.  When the library's runtime opens the GPU first, PyTorch's finds no
+    GPU afterwards ("No HIP GPUs are available"); initialised first, both work
+    (the binding hands PyTorch's device pointers to the library).  So when
+    PyTorch is importable and sees a GPU, it initialises before the library
+    loads."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def lib():
     """Load libsht_hip.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        _init_torch_device()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -295,3 +295,22 @@ def test_long_rings(im, prec, tol):
     for kind, o in enumerate(outs):
         assert rel(o.cpu().numpy(), ref.spec_to_grid(spec, kind=kind)) < tol
     p.close()
+
+
+def test_library_first_then_torch():
+    """A fresh process that creates a plan before any PyTorch GPU work, then
+    uses PyTorch on the GPU: the binding initialises PyTorch's device before
+    it loads the library (PyTorch ships its own HIP runtime)."""
+    import subprocess
+    import sys as _sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys; sys.path[:0] = [%r, %r]\n"
+            "import numpy as np, shtamd\n"
+            "p = shtamd.Plan(21, 64, 32, 4)\n"
+            "g = p.xya_wa(np.zeros((4, 22 * 22)))\n"
+            "import torch\n"
+            "t = torch.ones(8, device='cuda')\n"
+            "print('ok', float(t.sum()), float(np.abs(g).max()))\n") % (root, os.path.join(root, "dennou-ccm_amd", "py"))
+    say("library first, then torch")
+    r = subprocess.run([_sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0 and "ok 8.0 0.0" in r.stdout, (r.stdout, r.stderr[-2000:])
