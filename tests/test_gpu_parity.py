@@ -297,6 +297,33 @@ def test_long_rings(im, prec, tol):
     p.close()
 
 
+@pytest.mark.parametrize("im", [512, 1024, 2048])
+@pytest.mark.parametrize("prec,tol", [(shtamd.PREC_FP64, TOL), (shtamd.PREC_SPLIT_BF16X6, 1e-6)])
+def test_partial_field_groups_on_long_rings(im, prec, tol):
+    """On grids of 512 longitudes and more an FFT block holds 4, 2 or 1 of
+    the 8 fields of a Fourier-intermediate group; field counts that leave
+    the last group partly filled (blocks with no field at all, blocks that
+    share a group) must give each field the same result as the oracle, in
+    both directions, and a field's result must not depend on the batch."""
+    N, jm = 42, 64
+    ref = oracle.Plan(N, im, jm)
+    spec_all = synth.spectra(N, 17, cfg=21)
+    p = shtamd.Plan(N, im, jm, 17, precision=prec)
+    first = None
+    for nf in (1, 3, 5, 9, 17):
+        say("partial groups", im, prec, nf)
+        spec = spec_all[:, :nf]
+        g = gpu_inverse(p, spec, nf)
+        assert rel(g, ref.spec_to_grid(spec)) < tol, nf
+        s = gpu_forward(p, g, nf)
+        assert rel(s, ref.grid_to_spec(g)) < 2 * tol, nf
+        if first is None:
+            first = (g[0], s[:, 0])
+        else:  # field 0 alone and in every batch: the same numbers
+            assert np.array_equal(g[0], first[0]) and np.array_equal(s[:, 0], first[1]), nf
+    p.close()
+
+
 def test_library_first_then_torch():
     """A fresh process that creates a plan before any PyTorch GPU work, then
     uses PyTorch on the GPU: the binding initialises PyTorch's device before
