@@ -239,7 +239,8 @@ MIXED_TOL = {shtamd.PREC_SPLIT_BF16: 5e-5, shtamd.PREC_SCALED_FP16: 1e-6, shtamd
 
 @pytest.mark.parametrize("prec", sorted(MIXED_TOL))
 @pytest.mark.parametrize("N,im,jm,nf", [(85, 256, 128, 97), (21, 64, 32, 9), (170, 512, 256, 7),
-                                        (170, 512, 256, 69), (170, 512, 256, 200), (31, 64, 48, 17)])
+                                        (170, 512, 256, 69), (170, 512, 256, 200), (31, 64, 48, 17),
+                                        (106, 256, 240, 40)])
 def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
     """The mixed-precision shapes (one or two column tiles per workgroup, a
     wave per tile, or for the inverse the table held in registers while the
@@ -250,7 +251,8 @@ def test_mixed_shapes_bitwise_equal(prec, N, im, jm, nf):
     streaming shape (up to 8 column tiles per workgroup, loads 4 tiles ahead)
     runs single tiles (7 fields), chunks shorter than its load distance (69
     fields: 3 x 3 tiles) and longer ones with a short last chunk (200 fields:
-    7 + 7 + 7 + 4 tiles)."""
+    7 + 7 + 7 + 4 tiles); 240 latitudes (120 pairs) leave its last row tile
+    half filled."""
     say("mixed shapes", prec, N, nf)
     spec = synth.spectra(N, nf, cfg=14)
     ref = oracle.Plan(N, im, jm)
