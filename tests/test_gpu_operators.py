@@ -51,7 +51,9 @@ def gpu_grads(plan, spec, nf):
 
 CASES = [(21, 64, 32, 49, shtamd.PREC_FP64), (42, 128, 64, 79, shtamd.PREC_FP64),
          (85, 256, 128, 97, shtamd.PREC_FP64), (42, 128, 64, 20, shtamd.PREC_SPLIT_BF16X6),
-         (7, 16, 10, 5, shtamd.PREC_FP64)]
+         (7, 16, 10, 5, shtamd.PREC_FP64),
+         # FFT blocks that cover half or a quarter of an 8-field group
+         (42, 512, 64, 11, shtamd.PREC_FP64), (42, 1024, 64, 5, shtamd.PREC_FP64)]
 
 
 @pytest.mark.parametrize("N,im,jm,nf,prec", CASES)
